@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds per timestep and HBM GB/s of the implicit solve at 4096^2.
+
+Workload (one GPU): the seeded 4096x4096 deck (3 rectangles + 2 circles on
+[0,10]^2, numpy PCG64 seed 2), FP64 CG, tl_eps 1e-15, tl_max_iters 10000,
+dt 0.004, a field summary every step.  A "step" is one pass of the reference's
+diffuse() loop body: timestep + tea_leaf() solve + field_summary, run by the
+native host with every field resident in HBM.
+
+Prints one JSON line.  `value` is seconds per timestep (lower is better);
+`roofline` uses the algorithmic bytes of the CG calc_w kernel (48 B per cell:
+read r, p, Kx, Ky; write p, w) over its HIP-event-timed launches;
+`cpu_baseline` times the reference Fortran/OpenMP build (oracle/_ref) on the
+host cores on a bounded sample of the same deck.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+CG_W_BYTES_PER_CELL = 48.0      # fused calc_w: r, p, Kx, Ky in; p, w out
+CG_UR_BYTES_PER_CELL = 48.0     # calc_ur: u, p, r, w in; u, r out
+CG_ITER_BYTES_PER_CELL = 96.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_run(cells, solver, steps, warmup, workdir):
+    from tealeaf_ref_amd import _lib
+    from tealeaf_ref_amd.deckgen import bench_deck
+    from tealeaf_ref_amd.host import TeaHost
+
+    deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=10000,
+                      end_step=steps + warmup, summary_frequency=1)
+    deck_path = os.path.join(workdir, "tea.in")
+    with open(deck_path, "w") as f:
+        f.write(deck)
+    t0 = time.time()
+    h = TeaHost(deck_path, os.path.join(workdir, "tea.out"))
+    log(f"[bench] set-up {cells}^2 ({solver}) in {time.time() - t0:.1f} s")
+    for i in range(warmup):
+        h.step()
+        log(f"[bench] warmup step {i + 1}: {h.last()['iterations']:.0f} iterations")
+    _lib.call("tl_synchronize")
+    _lib.call("tl_profile_reset")
+    _lib.call("tl_profile_enable", 1)
+    iters = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        h.step()
+        it = h.last()["iterations"]
+        iters.append(it)
+        log(f"[bench] step {i + 1}: {it:.0f} iterations")
+    _lib.call("tl_synchronize")
+    elapsed = time.perf_counter() - t0
+    import ctypes as C
+    prof = {}
+    for cls in (0, 1):
+        n = C.c_long()
+        ms = C.c_double()
+        _lib.call("tl_profile_get", cls, C.byref(n), C.byref(ms))
+        prof[cls] = (n.value, ms.value)
+    _lib.call("tl_profile_enable", 0)
+    h.close()
+    return elapsed, iters, prof
+
+
+def cpu_baseline(cells, solver, gpu_iters_per_step, cap):
+    """Reference build (oracle/_ref) on the host cores: one timestep capped at
+    `cap` iterations, profiler_on -> Time Per It; falls back to the C oracle
+    (single thread) when the reference build is absent."""
+    from tealeaf_ref_amd.deckgen import bench_deck
+    deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=cap, end_step=1,
+                      summary_frequency=1, profiler=True)
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "tea_leaf")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if os.path.exists(ref_bin):
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "tea.in"), "w") as f:
+                f.write(deck)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            t0 = time.time()
+            subprocess.run([ref_bin], cwd=d, env=env, check=True, timeout=600,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            wall = time.time() - t0
+            text = open(os.path.join(d, "tea.out")).read()
+        m = re.search(r"Solve Time\s+(\S+)\s+Its\s+(\d+)\s+Time Per It\s+(\S+)", text)
+        tpi = float(m.group(3))
+        kind, cores = "reference", threads
+        sample = (f"{cells}^2 FP64 {solver.upper()}, 1 timestep capped at {cap} iterations, "
+                  f"OMP_NUM_THREADS={threads} (reference Fortran/OpenMP built from source by "
+                  f"oracle/ref.mk, {wall:.1f} s wall); s/timestep = Time Per It x the GPU run's "
+                  f"mean iterations per step")
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import tl_oracle as O
+        p = O.parse_deck(deck)
+        t0 = time.perf_counter()
+        rec = O.run(p)
+        wall = time.perf_counter() - t0
+        tpi = wall / max(1, rec[-1]["iterations"] + 1)
+        kind, cores = "port", 1
+        sample = f"C oracle, {cells}^2 FP64 {solver.upper()}, 1 step capped at {cap} iterations"
+    s_per_step = tpi * gpu_iters_per_step
+    return {"value": s_per_step, "unit": "s/timestep", "cores": cores, "kind": kind,
+            "sample": sample, "time_per_iteration_s": tpi,
+            "cell_iterations_per_s": cells * cells / tpi}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=4096)
+    ap.add_argument("--solver", default="cg", choices=["cg"])
+    ap.add_argument("--cpu-cap", type=int, default=100, help="iterations of the CPU sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the solver keeps the whole mesh as one chunk "
+              "on a single MI355X (multi-GPU decomposition is not built).", file=sys.stderr)
+        sys.exit(2)
+
+    with tempfile.TemporaryDirectory() as wd:
+        elapsed, iters, prof = gpu_run(args.cells, args.solver, args.steps, args.warmup, wd)
+    cells = args.cells * args.cells
+    s_per_step = elapsed / args.steps
+    exec_iters = sum(i + 1 for i in iters)        # printed count is n-1 at exit
+    mean_iters = sum(iters) / len(iters)
+    n_w, ms_w = prof[0]
+    n_ur, ms_ur = prof[1]
+    # achieved = algorithmic bytes of the executed launches / their event time
+    w_gbs = CG_W_BYTES_PER_CELL * cells * exec_iters / (ms_w * 1e-3) / 1e9 if ms_w else None
+    ur_gbs = CG_UR_BYTES_PER_CELL * cells * exec_iters / (ms_ur * 1e-3) / 1e9 if ms_ur else None
+    step_gbs = CG_ITER_BYTES_PER_CELL * cells * exec_iters / elapsed / 1e9
+    out = {
+        "metric": "s/timestep + HBM GB/s at 4096^2: FP64 CG vs FP32-refined PPCG @1e-10 residual",
+        "value": s_per_step,
+        "unit": "s/timestep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s_per_step * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded deck generator, no external data)",
+        "config": {"workload": f"{args.cells}x{args.cells} FP64 {args.solver.upper()} timestep, "
+                               "seeded deck (3 rectangles + 2 circles, seed 2), tl_eps 1e-15",
+                   "cells": cells, "solver": args.solver, "eps": 1e-15,
+                   "mean_iterations_per_step": mean_iters},
+        "hbm_gbs_step": step_gbs,
+        "cell_iterations_per_s": cells * exec_iters / elapsed,
+        "roofline": {"bound": "hbm", "kernel": "cg calc_w (w = A p, p = r + beta p, p.w)",
+                     "achieved": w_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": (w_gbs / PEAK_HBM_GBS) if w_gbs else None, "traffic": None,
+                     "bytes_per_cell": CG_W_BYTES_PER_CELL,
+                     "avg_launch_ms": ms_w / n_w if n_w else None, "launches": n_w,
+                     "calc_ur_achieved": ur_gbs,
+                     "calc_ur_avg_launch_ms": ms_ur / n_ur if n_ur else None},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap)
+        except Exception as e:  # report, do not fail the GPU measurement
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/*
+ * tealeaf_hip.h -- C ABI of the MI355X (gfx950) TeaLeaf solver library
+ * (libtealeaf_hip.so).  Plain C types only: ints, doubles and host pointers.
+ *
+ * One process drives one GPU holding the whole mesh as a single chunk.  The
+ * library owns every device buffer from tl_init() to tl_finalize(); no device
+ * pointer crosses this boundary.  Every entry point returns 0 on success or a
+ * TL_ERR_* code (message in tl_last_error()).  Entry points that return a
+ * scalar through a pointer synchronise the library's stream; all others are
+ * asynchronous and ordered on that stream.  Not thread safe.
+ *
+ * Host arrays use the reference layout: x fastest, bounds
+ * (x_min-halo : x_max+halo, y_min-halo : y_max+halo) with x_min = y_min = 1
+ * (FTNREF2D, ftocmacros.h:13; e.g. kernels/tea_leaf_cg_kernel.f90:131-132).
+ *
+ * Each declaration names the reference routine it replaces.  The reference
+ * plugs kernels in at the `IF (use_fortran_kernels)` branch of every driver
+ * wrapper (e.g. tea_leaf_cg.f90:61); a host binds these functions there.
+ */
+#ifndef TEALEAF_HIP_H
+#define TEALEAF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define TL_OK 0
+#define TL_ERR_HIP 1          /* a HIP runtime call failed                  */
+#define TL_ERR_STATE 2        /* not initialised / already initialised     */
+#define TL_ERR_ARG 3          /* bad argument (field id, size, option)     */
+#define TL_ERR_NONFINITE 4    /* a reduction produced NaN or Inf           */
+#define TL_ERR_EIGEN 5        /* eigenvalue estimate failed (tqli)         */
+
+/* ---- option values (data.f90:73-79) ----------------------------------- */
+#define TL_CONDUCTIVITY 1          /* tl_coefficient_density             */
+#define TL_RECIP_CONDUCTIVITY 2    /* tl_coefficient_inverse_density     */
+#define TL_PREC_NONE 1
+#define TL_PREC_JAC_DIAG 2
+#define TL_PREC_JAC_BLOCK 3
+
+/* ---- field ids: FIELD_* of data.f90:44-55, then the other arrays of
+ *      field_type (definitions.f90:136-174) ------------------------------ */
+#define TL_FIELD_DENSITY 1
+#define TL_FIELD_ENERGY0 2
+#define TL_FIELD_ENERGY1 3
+#define TL_FIELD_U 4
+#define TL_FIELD_P 5
+#define TL_FIELD_SD 6
+#define TL_FIELD_R 7
+#define TL_FIELD_Z 8
+#define TL_FIELD_KX 9
+#define TL_FIELD_KY 10
+#define TL_FIELD_DI 11
+#define TL_FIELD_U0 12
+#define TL_FIELD_W 13
+#define TL_FIELD_R_STORE 14
+#define TL_FIELD_MI 15
+#define TL_FIELD_UTEMP 16
+#define TL_FIELD_RTEMP 17
+#define TL_FIELD_VOLUME 18
+#define TL_FIELD_CP 19
+#define TL_FIELD_BFP 20
+#define TL_NUM_FIELDS 20
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Allocates and zeroes every field for an nx*ny chunk with `halo` halo
+ * cells (build_field.f90:43-243) on HIP device `device`.
+ * Replaces: build_field() + the ALLOCATEs of chunk%tiles(t)%field. */
+int tl_init(int nx, int ny, int halo, int device);
+
+/* Frees everything.  (The reference never deallocates, tea_leaf.f90:70.) */
+int tl_finalize(void);
+
+/* Last error message (static storage, valid until the next call). */
+const char *tl_last_error(void);
+
+/* Solver options that the reference keeps in definitions_module globals:
+ * coefficient (tea_solve.f90:75), tl_preconditioner_type, reflective_boundary
+ * (read_input.f90:207-233), chunk%halo_exchange_depth. */
+int tl_set_options(int coefficient, int preconditioner, int reflective);
+
+/* tl_ppcg_active (definitions.f90:71), toggled by tea_solve.f90:79,209. */
+int tl_set_ppcg_active(int active);
+
+/* Copy a host array with `host_halo` halo cells to/from a device field.
+ * Only the cells inside both halos are copied.  volume uses host_halo = 2
+ * (kernels/field_summary_kernel.f90:36); cp and bfp use host_halo = 0. */
+int tl_upload(int field_id, const double *host, int host_halo);
+int tl_download(int field_id, double *host, int host_halo);
+
+/* Block until all queued work is done. */
+int tl_synchronize(void);
+
+/* ---- per-step kernels (tea_leaf_common.f90) ------------------------------ */
+
+/* u = u0 = energy1*density; Kx, Ky from density or 1/density; zero-flux
+ * faces unless reflective; Di; preconditioner set-up; w = A u; r = u - w.
+ * Replaces tea_leaf_init_common (tea_leaf_common.f90:12-61) ->
+ * tea_leaf_common_init_kernel (kernels/tea_leaf_common_kernel.f90:26-177). */
+int tl_common_init(double rx, double ry, const int zero_boundary[4]);
+
+/* r = u0 - A u.  Replaces tea_leaf_calc_residual (tea_leaf_common.f90:63-92). */
+int tl_calc_residual(void);
+
+/* norm = sum(arr^2), arr = u0 (which = 0) or r (which = 1).
+ * Replaces tea_leaf_calc_2norm (tea_leaf_common.f90:94-138). */
+int tl_calc_2norm(int which, double *norm);
+
+/* energy1 = u/density.  Replaces tea_leaf_finalise (tea_leaf_common.f90:140-163). */
+int tl_finalise(void);
+
+/* energy1 = energy0 on the interior.  Replaces set_field (set_field.f90). */
+int tl_set_field(void);
+
+/* Reflective external-halo update of the fields flagged in fields[0..10]
+ * (FIELD_* order) to `depth`; no-op unless reflective.  The FIELD_R slot
+ * updates rtemp, as update_boundary does (update_halo.f90:79).
+ * Replaces update_halo (update_halo.f90:32-47) on one chunk. */
+int tl_update_halo(const int fields[11], int depth);
+
+/* vol = sum(volume), mass = sum(volume*density), ie = sum(mass*energy1),
+ * temp = sum(mass*u).  Replaces field_summary_kernel
+ * (kernels/field_summary_kernel.f90:26-64) and the tea_sum calls of
+ * field_summary.f90:88-91. */
+int tl_field_summary(double *vol, double *mass, double *ie, double *temp);
+
+/* ---- CG (tea_leaf_cg.f90) ---------------------------------------------- */
+int tl_cg_init(double *rro);                  /* tea_leaf_cg_init  :11-50   */
+int tl_cg_calc_w(double *pw);                 /* tea_leaf_cg_calc_w :52-88  */
+int tl_cg_calc_ur(double alpha, double *rrn); /* tea_leaf_cg_calc_ur :90-131 */
+int tl_cg_calc_p(double beta);                /* tea_leaf_cg_calc_p :133-158 */
+
+/* ---- Chebyshev (tea_leaf_cheby.f90) --------------------------------------- */
+int tl_cheby_init(double theta);              /* tea_leaf_cheby_init :13-49 */
+/* One iteration with coefficients ch_alphas(step), ch_betas(step).
+ * Replaces tea_leaf_cheby_iterate (tea_leaf_cheby.f90:51-88). */
+int tl_cheby_iterate(double alpha, double beta);
+
+/* ---- PPCG (tea_leaf_ppcg.f90) -------------------------------------------- */
+int tl_ppcg_init(int step, double *rro);      /* tea_leaf_ppcg_init :72-116 */
+int tl_ppcg_init_sd(double theta);            /* tea_leaf_ppcg_init_sd :33-68 */
+/* Inner sweeps first..first+count-1 with coefficients alphas[i-1], betas[i-1]
+ * (1-based sweep index i).  Replaces the sweep loop of
+ * tea_leaf_run_ppcg_inner_steps (tea_leaf_ppcg.f90:231-424), including the
+ * reflective halo refresh of sd after every sweep (:316-339). */
+int tl_ppcg_inner(const double *alphas, const double *betas, int first, int count);
+int tl_ppcg_store_r(void);                    /* tea_leaf_ppcg_store_r :433-454 */
+int tl_ppcg_calc_zrnorm(double *rro);         /* tea_leaf_ppcg_calc_zrnorm :118-148 */
+int tl_ppcg_calc_rrn(double *rrn);            /* tea_leaf_ppcg_calc_rrn :456-486 */
+int tl_ppcg_update_z(void);                   /* tea_leaf_ppcg_update_z :150-172 */
+
+/* ---- Jacobi (tea_leaf_jacobi.f90) -------------------------------------- */
+int tl_jacobi_solve(double *error);           /* tea_leaf_jacobi_solve :11-46 */
+
+/* ---- fused solver loops (device-resident alpha/beta and exit test) -------
+ * Each runs consecutive iterations of the tea_leaf() DO loop
+ * (tea_solve.f90:152-414) for one solver branch, keeping the reductions,
+ * alpha = rro/pw, beta = rrn/rro and the exit test |sqrt(rrn)| < eps*r0 on
+ * the device, so the host synchronises once per batch instead of twice per
+ * iteration.  Iteration counts, the recorded CG alphas/betas and the final
+ * error are those of the reference loop. */
+typedef struct {
+  /* in */
+  int n_first;          /* first iteration number n to execute (1-based)      */
+  int n_last;           /* last iteration allowed (tl_max_iters)              */
+  double eps_abs;       /* eps * initial_residual (tea_solve.f90:410)          */
+  int check_switch;     /* test the CG->Cheby/PPCG switch (tea_solve.f90:167)  */
+  double epslim;        /* tl_ch_cg_epslim                                     */
+  int presteps;         /* tl_ch_cg_presteps                                   */
+  /* in/out */
+  double rro;           /* rro entering n_first / after the last iteration    */
+  double old_error;     /* old_error entering n_first / on exit               */
+  /* out */
+  int n_end;            /* iteration at which the loop stopped                */
+  int reason;           /* TL_LOOP_* below                                    */
+  double error;         /* sqrt|rrn| (or sqrt|error|) of iteration n_end      */
+} tl_loop_state;
+
+#define TL_LOOP_CONVERGED 1  /* iteration n_end met the exit test            */
+#define TL_LOOP_SWITCH 2     /* switch condition true at the top of n_end    */
+#define TL_LOOP_MAXITERS 3   /* n_end == n_last without meeting the test     */
+
+/* CG branch (tea_solve.f90:341-369).  cg_alphas/cg_betas are host arrays
+ * indexed by n (entry n written for every executed iteration), as
+ * cg_alphas(n)/cg_betas(n) at :353,363.  Either may be NULL. */
+int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas);
+
+/* PPCG outer iterations after the switch (tea_solve.f90:289-331): calc_w,
+ * store_r, zrnorm, calc_ur, `inner` sweeps with coefficients alphas/betas
+ * (length inner), calc_rrn, calc_p.  inner_iters is incremented by `inner`
+ * per executed outer iteration. */
+int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const double *betas,
+                 int inner, double theta, int *inner_iters);
+
+/* Jacobi branch (tea_solve.f90:375-379). */
+int tl_jacobi_loop(tl_loop_state *st);
+
+/* Chebyshev iterations n_first..n_last with coefficient index
+ * step_first.. (tea_solve.f90:265-279), queued without synchronisation;
+ * the residual norm is computed only on the last one when want_norm != 0. */
+int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int step_first,
+                 int count, int want_norm, double *norm);
+
+/* ---- timing of the dominant kernels (HIP events on the library stream) --- */
+/* Enables per-launch HIP-event timing of the solver kernels. */
+int tl_profile_enable(int on);
+/* Returns launch count and total milliseconds for a kernel class:
+ * 0 = CG calc_w, 1 = CG calc_ur, 2 = PPCG inner sweep, 3 = Chebyshev iterate,
+ * 4 = Jacobi, 5 = PPCG calc_w, 6 = PPCG calc_ur. */
+int tl_profile_get(int kernel_class, long *launches, double *total_ms);
+int tl_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEALEAF_HIP_H */

@@ -1,0 +1,169 @@
+// Internal definitions shared by the translation units of libtealeaf_hip.so.
+//
+// Device layout of every field: rows of `pitch` doubles, 256-byte aligned;
+// interior cell (j,k) (1-based, as in the reference) lives at
+//   base + (k-1+h)*pitch + PADL + (j-1)
+// so each interior row starts on a 256-byte boundary and halo cells j<1 / j>nx
+// and k<1 / k>ny are addressable up to depth h.  Kernels take a pointer to cell
+// (1,1) and index it with 0-based (x, y) = (j-1, k-1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tealeaf_hip.h"
+
+namespace tl {
+
+constexpr int PADL = 32;          // doubles before interior column 1 (256 B)
+constexpr int BLOCK = 256;        // threads per block for streaming kernels
+constexpr int WAVE = 64;          // CDNA wavefront width
+constexpr int MAX_GRID = 2048;    // max blocks of a reduction kernel (= partials)
+constexpr int NUM_XCD = 8;        // MI355X: 8 XCDs, blocks dealt round robin
+
+// Boundary-read modes of the 5-point operator.
+//  MIRROR: a neighbour outside the interior reads the mirrored interior cell,
+//          which equals the reflective halo the reference refreshes before
+//          the read, and is multiplied by a zero face coefficient otherwise.
+//  PHYS:   read the stored halo cell (the reference's exact state, including
+//          halos it leaves stale).
+//  ZERO:   read 0 (the un = vector_r halo of the Jacobi kernel, which the
+//          reference never writes).
+enum Bnd { MIRROR = 0, PHYS = 1, ZERO = 2 };
+
+struct Field {
+  double *base = nullptr;  // allocation
+  double *p11 = nullptr;   // cell (1,1)
+};
+
+struct State {
+  bool init = false;
+  int nx = 0, ny = 0, h = 1, device = 0;
+  long pitch = 0;          // doubles per row
+  long rows = 0;           // ny + 2h
+  size_t field_bytes = 0;
+  hipStream_t stream = nullptr;
+
+  // options
+  int coefficient = TL_CONDUCTIVITY;
+  int precond = TL_PREC_NONE;
+  int reflective = 0;
+  int ppcg_active = 0;
+  double rx = 0.0, ry = 0.0;
+
+  // fields (two buffers for the ones that are ping-ponged)
+  Field density, energy0, energy1, u0, r, r_store, Mi, w, z, utemp, rtemp, Di,
+      Kx, Ky, volume, cp, bfp;
+  Field u[2], p[2], sd[2];
+  int ucur = 0, pcur = 0, sdcur = 0;
+
+  // reductions: partial sums per block, and small device scalars
+  double *partials = nullptr;  // NPART arrays of MAX_GRID doubles
+  double *dscal = nullptr;     // scratch scalars (results of finish kernels)
+  double *hscal = nullptr;     // pinned host mirror of dscal
+  void *ctl = nullptr;         // loop control block (device)
+  void *hctl = nullptr;        // pinned host copies of the control block
+  double *hist = nullptr;      // loop history: rro, alpha, beta (3 x hist_len)
+  long hist_len = 0;
+
+  // profiling
+  bool prof = false;
+  struct Ev { hipEvent_t a, b; int cls; };
+  std::vector<Ev> ev_pending;
+  std::vector<hipEvent_t> ev_pool;
+  long prof_count[8] = {0};
+  double prof_ms[8] = {0};
+
+  std::string err;
+};
+
+State &S();
+
+// partial-sum slots
+enum Part { P0 = 0, P1, P2, P3, P4, P5, NPART };
+inline double *part(int slot) { return S().partials + (long)slot * MAX_GRID; }
+
+// set the error string and return the code
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define TL_HIP(call)                                         \
+  do {                                                       \
+    hipError_t e__ = (call);                                 \
+    if (e__ != hipSuccess) return ::tl::hip_fail(e__, #call); \
+  } while (0)
+
+#define TL_REQUIRE_INIT()                                             \
+  do {                                                                \
+    if (!::tl::S().init) return ::tl::fail(TL_ERR_STATE, "tl_init has not been called"); \
+  } while (0)
+
+// launch geometry for streaming kernels over the nx*ny interior
+struct Grid {
+  int tiles_x = 0, tiles_y = 0, rows_per_tile = 1;
+  int blocks = 0;          // launched blocks (<= MAX_GRID, multiple of NUM_XCD)
+  long tiles = 0;
+};
+Grid grid_for(int nx, int ny);
+
+// profiling helpers (no-ops unless enabled)
+void prof_begin(int cls, hipEvent_t *a);
+void prof_end(int cls, hipEvent_t a);
+
+// finish a reduction: sum `n` partials of `slot` (deterministically) into
+// dscal[idx]; returns after copying it to the host (synchronises)
+int finish_sums(const int *slots, int nslots, int n, double *out);
+
+}  // namespace tl
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+namespace tl {
+namespace dev {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+// Deterministic block sum: fixed lane tree per wave, then waves in order.
+// Every thread returns the same total.  `lds` holds BLOCK/WAVE doubles.
+__device__ inline double block_sum(double v, double *lds) {
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / WAVE; ++i) t += lds[i];
+  return t;
+}
+
+// Sum of n partials, identical bits in every block of every launch.
+__device__ inline double sum_partials(const double *__restrict__ p, int n, double *lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += BLOCK) s += p[i];
+  return block_sum(s, lds);
+}
+
+// Tile walk shared by the streaming kernels: blocks are remapped so that the
+// blocks of one XCD own a contiguous band of tiles (halo rows stay in one L2).
+struct TileIter {
+  long t, t_end;
+  __device__ TileIter(long tiles, int blocks) {
+    const int b = blockIdx.x;
+    const int lb = (blocks % NUM_XCD == 0) ? (b % NUM_XCD) * (blocks / NUM_XCD) + b / NUM_XCD : b;
+    const long base = tiles / blocks, rem = tiles % blocks;
+    t = lb * base + (lb < rem ? lb : rem);
+    t_end = t + base + (lb < rem ? 1 : 0);
+  }
+};
+
+}  // namespace dev
+}  // namespace tl

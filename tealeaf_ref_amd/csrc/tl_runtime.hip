@@ -1,0 +1,331 @@
+// Lifecycle, field storage, host<->device copies, reduction finishing and
+// kernel timing of libtealeaf_hip.so.
+#include <cstdio>
+#include <cstring>
+
+#include "tl_internal.hpp"
+#include "tl_stencil.hpp"
+
+namespace tl {
+
+State &S() {
+  static State s;
+  return s;
+}
+
+int fail(int code, const std::string &msg) {
+  S().err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  S().err = std::string(what) + ": " + hipGetErrorString(e);
+  return TL_ERR_HIP;
+}
+
+Grid grid_for(int nx, int ny) {
+  Grid g;
+  g.tiles_x = (nx + BLOCK - 1) / BLOCK;
+  long r = ((long)g.tiles_x * ny) / MAX_GRID;
+  if (r < 1) r = 1;
+  if (r > 32) r = 32;
+  g.rows_per_tile = (int)r;
+  g.tiles_y = (ny + g.rows_per_tile - 1) / g.rows_per_tile;
+  g.tiles = (long)g.tiles_x * g.tiles_y;
+  long b = g.tiles < MAX_GRID ? g.tiles : MAX_GRID;
+  if (b >= NUM_XCD) b -= b % NUM_XCD;
+  g.blocks = (int)b;
+  return g;
+}
+
+// --------------------------------------------------------------------------
+// kernel timing
+// --------------------------------------------------------------------------
+void prof_begin(int cls, hipEvent_t *a) {
+  State &s = S();
+  *a = nullptr;
+  if (!s.prof) return;
+  if (s.ev_pool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    s.ev_pool.push_back(e);
+  }
+  *a = s.ev_pool.back();
+  s.ev_pool.pop_back();
+  (void)hipEventRecord(*a, s.stream);
+  (void)cls;
+}
+
+void prof_end(int cls, hipEvent_t a) {
+  State &s = S();
+  if (!s.prof || !a) return;
+  hipEvent_t b;
+  if (s.ev_pool.empty()) {
+    if (hipEventCreate(&b) != hipSuccess) return;
+  } else {
+    b = s.ev_pool.back();
+    s.ev_pool.pop_back();
+  }
+  (void)hipEventRecord(b, s.stream);
+  s.ev_pending.push_back({a, b, cls});
+  // keep the number of outstanding events bounded
+  if (s.ev_pending.size() >= 4096) {
+    (void)hipStreamSynchronize(s.stream);
+    for (auto &ev : s.ev_pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) {
+        s.prof_ms[ev.cls] += ms;
+        s.prof_count[ev.cls] += 1;
+      }
+      s.ev_pool.push_back(ev.a);
+      s.ev_pool.push_back(ev.b);
+    }
+    s.ev_pending.clear();
+  }
+}
+
+static void prof_drain() {
+  State &s = S();
+  if (s.ev_pending.empty()) return;
+  (void)hipStreamSynchronize(s.stream);
+  for (auto &ev : s.ev_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) {
+      s.prof_ms[ev.cls] += ms;
+      s.prof_count[ev.cls] += 1;
+    }
+    s.ev_pool.push_back(ev.a);
+    s.ev_pool.push_back(ev.b);
+  }
+  s.ev_pending.clear();
+}
+
+// --------------------------------------------------------------------------
+// reduction finish: one block sums the partials of each requested slot
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_finish(const double *__restrict__ partials, int n,
+                                                  const int4 slots, int nslots,
+                                                  double *__restrict__ out) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int sl[4] = {slots.x, slots.y, slots.z, slots.w};
+  for (int i = 0; i < nslots; ++i) {
+    const double v = dev::sum_partials(partials + (long)sl[i] * MAX_GRID, n, lds);
+    if (threadIdx.x == 0) out[i] = v;
+    __syncthreads();
+  }
+}
+
+int finish_sums(const int *slots, int nslots, int n, double *out) {
+  State &s = S();
+  int4 sl = make_int4(slots[0], nslots > 1 ? slots[1] : 0, nslots > 2 ? slots[2] : 0,
+                      nslots > 3 ? slots[3] : 0);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, s.stream, s.partials, n, sl, nslots,
+                     s.dscal);
+  TL_HIP(hipGetLastError());
+  TL_HIP(hipMemcpyAsync(s.hscal, s.dscal, sizeof(double) * nslots, hipMemcpyDeviceToHost,
+                        s.stream));
+  TL_HIP(hipStreamSynchronize(s.stream));
+  for (int i = 0; i < nslots; ++i) out[i] = s.hscal[i];
+  return TL_OK;
+}
+
+}  // namespace tl
+
+using namespace tl;
+
+// --------------------------------------------------------------------------
+// C ABI: lifecycle
+// --------------------------------------------------------------------------
+static int alloc_field(Field &f) {
+  State &s = S();
+  TL_HIP(hipMalloc((void **)&f.base, s.field_bytes));
+  TL_HIP(hipMemsetAsync(f.base, 0, s.field_bytes, s.stream));
+  f.p11 = f.base + (long)s.h * s.pitch + PADL;
+  return TL_OK;
+}
+
+static void free_field(Field &f) {
+  if (f.base) (void)hipFree(f.base);
+  f.base = f.p11 = nullptr;
+}
+
+static Field *all_fields(int i) {
+  State &s = S();
+  Field *list[] = {&s.density, &s.energy0, &s.energy1, &s.u0, &s.r, &s.r_store, &s.Mi,
+                   &s.w, &s.z, &s.utemp, &s.rtemp, &s.Di, &s.Kx, &s.Ky, &s.volume,
+                   &s.cp, &s.bfp, &s.u[0], &s.u[1], &s.p[0], &s.p[1], &s.sd[0], &s.sd[1]};
+  const int n = sizeof(list) / sizeof(list[0]);
+  return i < n ? list[i] : nullptr;
+}
+
+extern "C" int tl_finalize(void) {
+  State &s = S();
+  if (!s.init) return TL_OK;
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
+  for (int i = 0; Field *f = all_fields(i); ++i) free_field(*f);
+  if (s.partials) (void)hipFree(s.partials);
+  if (s.dscal) (void)hipFree(s.dscal);
+  if (s.hscal) (void)hipHostFree(s.hscal);
+  if (s.ctl) (void)hipFree(s.ctl);
+  if (s.hctl) (void)hipHostFree(s.hctl);
+  if (s.hist) (void)hipFree(s.hist);
+  for (auto &ev : s.ev_pending) {
+    (void)hipEventDestroy(ev.a);
+    (void)hipEventDestroy(ev.b);
+  }
+  for (auto e : s.ev_pool) (void)hipEventDestroy(e);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  s = State();
+  return TL_OK;
+}
+
+extern "C" int tl_init(int nx, int ny, int halo, int device) {
+  State &s = S();
+  if (s.init) return fail(TL_ERR_STATE, "tl_init called twice without tl_finalize");
+  if (nx < 1 || ny < 1 || halo < 1 || halo > PADL)
+    return fail(TL_ERR_ARG, "tl_init: need nx, ny >= 1 and 1 <= halo <= 32");
+  int ndev = 0;
+  TL_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(TL_ERR_ARG, "tl_init: no such HIP device");
+  TL_HIP(hipSetDevice(device));
+  s.nx = nx;
+  s.ny = ny;
+  s.h = halo;
+  s.device = device;
+  s.pitch = ((long)PADL + nx + halo + 31) / 32 * 32;
+  s.rows = (long)ny + 2L * halo;
+  s.field_bytes = sizeof(double) * (size_t)(s.pitch * s.rows + 64);
+  TL_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+  s.init = true;  // so that tl_finalize cleans up on a failure below
+  int rc;
+  for (int i = 0; Field *f = all_fields(i); ++i)
+    if ((rc = alloc_field(*f)) != TL_OK) {
+      std::string m = s.err;
+      tl_finalize();
+      return fail(rc, m);
+    }
+  TL_HIP(hipMalloc((void **)&s.partials, sizeof(double) * MAX_GRID * NPART));
+  TL_HIP(hipMemsetAsync(s.partials, 0, sizeof(double) * MAX_GRID * NPART, s.stream));
+  TL_HIP(hipMalloc((void **)&s.dscal, sizeof(double) * 64));
+  TL_HIP(hipHostMalloc((void **)&s.hscal, sizeof(double) * 64, hipHostMallocDefault));
+  TL_HIP(hipStreamSynchronize(s.stream));
+  return TL_OK;
+}
+
+extern "C" const char *tl_last_error(void) { return S().err.c_str(); }
+
+extern "C" int tl_set_options(int coefficient, int preconditioner, int reflective) {
+  TL_REQUIRE_INIT();
+  State &s = S();
+  if (coefficient != TL_CONDUCTIVITY && coefficient != TL_RECIP_CONDUCTIVITY)
+    return fail(TL_ERR_ARG, "tea_leaf: unknown coefficient option");
+  if (preconditioner < TL_PREC_NONE || preconditioner > TL_PREC_JAC_BLOCK)
+    return fail(TL_ERR_ARG, "tl_set_options: invalid preconditioner type");
+  if (preconditioner == TL_PREC_JAC_BLOCK && s.h > 1)
+    return fail(TL_ERR_ARG,
+                "read_input: Unable to use nonstandard halo depth with block jacobi preconditioner");
+  s.coefficient = coefficient;
+  s.precond = preconditioner;
+  s.reflective = reflective ? 1 : 0;
+  return TL_OK;
+}
+
+extern "C" int tl_set_ppcg_active(int active) {
+  TL_REQUIRE_INIT();
+  S().ppcg_active = active ? 1 : 0;
+  return TL_OK;
+}
+
+extern "C" int tl_synchronize(void) {
+  TL_REQUIRE_INIT();
+  TL_HIP(hipStreamSynchronize(S().stream));
+  return TL_OK;
+}
+
+namespace tl {
+Field *field_by_id(int id) {
+  State &s = S();
+  switch (id) {
+    case TL_FIELD_DENSITY: return &s.density;
+    case TL_FIELD_ENERGY0: return &s.energy0;
+    case TL_FIELD_ENERGY1: return &s.energy1;
+    case TL_FIELD_U: return &s.u[s.ucur];
+    case TL_FIELD_P: return &s.p[s.pcur];
+    case TL_FIELD_SD: return &s.sd[s.sdcur];
+    case TL_FIELD_R: return &s.r;
+    case TL_FIELD_Z: return &s.z;
+    case TL_FIELD_KX: return &s.Kx;
+    case TL_FIELD_KY: return &s.Ky;
+    case TL_FIELD_DI: return &s.Di;
+    case TL_FIELD_U0: return &s.u0;
+    case TL_FIELD_W: return &s.w;
+    case TL_FIELD_R_STORE: return &s.r_store;
+    case TL_FIELD_MI: return &s.Mi;
+    case TL_FIELD_UTEMP: return &s.utemp;
+    case TL_FIELD_RTEMP: return &s.rtemp;
+    case TL_FIELD_VOLUME: return &s.volume;
+    case TL_FIELD_CP: return &s.cp;
+    case TL_FIELD_BFP: return &s.bfp;
+    default: return nullptr;
+  }
+}
+}  // namespace tl
+
+// Copy the rectangle of cells inside both halos.
+static int copy_field(int id, double *host, int host_halo, bool up) {
+  TL_REQUIRE_INIT();
+  State &s = S();
+  Field *f = field_by_id(id);
+  if (!f) return fail(TL_ERR_ARG, "unknown field id " + std::to_string(id));
+  if (!host) return fail(TL_ERR_ARG, "null host pointer");
+  if (host_halo < 0) return fail(TL_ERR_ARG, "negative host halo");
+  const int hh = host_halo < s.h ? host_halo : s.h;
+  const long hw = (long)s.nx + 2L * host_halo;           // host row length
+  const long cols = (long)s.nx + 2L * hh, rows = (long)s.ny + 2L * hh;
+  double *hp = host + (long)(host_halo - hh) + (long)(host_halo - hh) * hw;
+  double *dp = f->p11 - hh - (long)hh * s.pitch;
+  if (up)
+    TL_HIP(hipMemcpy2DAsync(dp, s.pitch * sizeof(double), hp, hw * sizeof(double),
+                            cols * sizeof(double), rows, hipMemcpyHostToDevice, s.stream));
+  else
+    TL_HIP(hipMemcpy2DAsync(hp, hw * sizeof(double), dp, s.pitch * sizeof(double),
+                            cols * sizeof(double), rows, hipMemcpyDeviceToHost, s.stream));
+  TL_HIP(hipStreamSynchronize(s.stream));
+  return TL_OK;
+}
+
+extern "C" int tl_upload(int field_id, const double *host, int host_halo) {
+  return copy_field(field_id, const_cast<double *>(host), host_halo, true);
+}
+
+extern "C" int tl_download(int field_id, double *host, int host_halo) {
+  return copy_field(field_id, host, host_halo, false);
+}
+
+// --------------------------------------------------------------------------
+// C ABI: profiling
+// --------------------------------------------------------------------------
+extern "C" int tl_profile_enable(int on) {
+  TL_REQUIRE_INIT();
+  S().prof = on != 0;
+  return TL_OK;
+}
+
+extern "C" int tl_profile_get(int cls, long *launches, double *total_ms) {
+  TL_REQUIRE_INIT();
+  if (cls < 0 || cls >= 8) return fail(TL_ERR_ARG, "bad kernel class");
+  prof_drain();
+  if (launches) *launches = S().prof_count[cls];
+  if (total_ms) *total_ms = S().prof_ms[cls];
+  return TL_OK;
+}
+
+extern "C" int tl_profile_reset(void) {
+  TL_REQUIRE_INIT();
+  prof_drain();
+  for (int i = 0; i < 8; ++i) {
+    S().prof_count[i] = 0;
+    S().prof_ms[i] = 0.0;
+  }
+  return TL_OK;
+}

@@ -1,0 +1,1247 @@
+!> Native TeaLeaf host for one MI355X: reads the reference's tea.in deck,
+!> generates the chunk, runs the diffuse() step loop and the tea_leaf()
+!> implicit-solve driver, and writes tea.out in the reference's formats.
+!> All field work runs in libtealeaf_hip.so through the C ABI (tl_hip_api);
+!> this module only holds scalars, the deck and the generation arrays.
+!>
+!> Reference counterparts: read_input.f90 + parse.f90 (deck), start.f90 +
+!> initialise_chunk.f90 + generate_chunk.f90 (setup), diffuse.f90 (step loop),
+!> tea_solve.f90 (driver), field_summary.f90 (summary + QA), timestep.f90.
+MODULE tea_host
+
+  USE ISO_C_BINDING
+  USE tl_hip_api
+  IMPLICIT NONE
+
+  ! ---- constants (data.f90) -------------------------------------------------
+  REAL(KIND=8), PARAMETER :: g_version = REAL(1.403, 8)
+  INTEGER, PARAMETER :: g_ibig = 640000
+  REAL(KIND=8), PARAMETER :: g_small = REAL(1.0e-16, 8)
+  INTEGER, PARAMETER :: FIELD_DENSITY = 1, FIELD_ENERGY0 = 2, FIELD_ENERGY1 = 3, FIELD_U = 4, &
+                        FIELD_P = 5, FIELD_SD = 6, FIELD_R = 7, FIELD_Z = 8, FIELD_KX = 9,   &
+                        FIELD_KY = 10, FIELD_DI = 11, NUM_FIELDS = 11
+  INTEGER, PARAMETER :: CONDUCTIVITY = 1, RECIP_CONDUCTIVITY = 2
+  INTEGER, PARAMETER :: TL_PREC_NONE = 1, TL_PREC_JAC_DIAG = 2, TL_PREC_JAC_BLOCK = 3
+  INTEGER, PARAMETER :: g_rect = 1, g_circ = 2, g_point = 3
+  INTEGER, PARAMETER :: LINE_LEN = 100
+
+  TYPE state_type
+    LOGICAL :: defined = .FALSE.
+    REAL(KIND=8) :: density = 0.0_8, energy = 0.0_8
+    REAL(KIND=8) :: xmin = 0.0_8, ymin = 0.0_8, xmax = 0.0_8, ymax = 0.0_8, radius = 0.0_8
+    INTEGER :: geometry = 0
+  END TYPE state_type
+
+  TYPE profiler_type
+    REAL(KIND=8) :: timestep = 0, visit = 0, summary = 0, halo_exchange = 0, halo_update = 0, &
+                    internal_halo_update = 0, tea_init = 0, tea_solve = 0, tea_reset = 0,     &
+                    set_field = 0, dot_product = 0
+  END TYPE profiler_type
+
+  ! ---- deck values (read_input.f90) -------------------------------------------
+  REAL(KIND=8) :: xmin, ymin, xmax, ymax, end_time, dtinit, eps, tl_ch_cg_epslim
+  INTEGER :: x_cells, y_cells, end_step, visit_frequency, summary_frequency, max_iters
+  INTEGER :: tl_ch_cg_presteps, tl_ppcg_inner_steps, tl_preconditioner_type, coefficient
+  INTEGER :: halo_depth, test_problem, number_of_states
+  LOGICAL :: tl_check_result, reflective_boundary, profiler_on, verbose_on
+  LOGICAL :: tl_use_cg, tl_use_chebyshev, tl_use_ppcg, tl_use_jacobi
+  TYPE(state_type), ALLOCATABLE :: states(:)
+
+  ! ---- run state ---------------------------------------------------------------
+  INTEGER :: step = 0
+  REAL(KIND=8) :: time = 0.0_8, dt = 0.0_8, dx = 0.0_8, dy = 0.0_8
+  LOGICAL :: complete = .FALSE., host_ready = .FALSE., out_is_file = .FALSE.
+  INTEGER :: g_out = 6
+  TYPE(profiler_type) :: profiler
+  REAL(KIND=8) :: timerstart = 0.0_8, first_step_t = 0.0_8, second_step_t = 0.0_8
+  ! SAVEd solver state of tea_solve.f90:55,58
+  REAL(KIND=8) :: eigmin = 0.0_8, eigmax = 0.0_8, theta = 0.0_8, cn = 0.0_8
+  LOGICAL :: first = .TRUE.
+  ! results of the last step, read through tea_host_last()
+  INTEGER :: last_iterations = 0, last_ppcg_inner = 0, last_switch = 0, last_est_itc = 0
+  REAL(KIND=8) :: last_cond_error = 0.0_8, last_exact_error = 0.0_8, last_solve_time = 0.0_8
+  REAL(KIND=8) :: last_summary(5) = 0.0_8
+  INTEGER :: last_summary_step = -1
+
+  ! ---- deck text ------------------------------------------------------------------
+  CHARACTER(LEN=LINE_LEN), ALLOCATABLE :: raw_lines(:), deck_lines(:)
+  INTEGER :: n_raw = 0, n_deck = 0, deck_pos = 0
+  CHARACTER(LEN=LINE_LEN + 2) :: cur_line
+  CHARACTER(LEN=64) :: here
+
+CONTAINS
+
+  ! ===========================================================================
+  ! errors and timing
+  ! ===========================================================================
+  SUBROUTINE report_error(location, msg)
+    CHARACTER(LEN=*), INTENT(IN) :: location, msg
+    WRITE(0, '(a)') ' '
+    WRITE(0, '(a)') ' Error from '//TRIM(location)//':'
+    WRITE(0, '(a)') ' '//TRIM(msg)
+    WRITE(0, '(a)') ' TEA is terminating.'
+    IF (g_out /= 6) THEN
+      WRITE(g_out, '(a)') ' '
+      WRITE(g_out, '(a)') ' Error from '//TRIM(location)//':'
+      WRITE(g_out, '(a)') ' '//TRIM(msg)
+      WRITE(g_out, '(a)') ' TEA is terminating.'
+      FLUSH(g_out)
+    ENDIF
+    ERROR STOP 1
+  END SUBROUTINE report_error
+
+  SUBROUTINE hip_check(rc, where)
+    INTEGER(c_int), INTENT(IN) :: rc
+    CHARACTER(LEN=*), INTENT(IN) :: where
+    CHARACTER(KIND=c_char), POINTER :: cmsg(:)
+    CHARACTER(LEN=512) :: msg
+    INTEGER :: i
+    IF (rc == TL_OK) RETURN
+    CALL C_F_POINTER(tl_last_error(), cmsg, [512])
+    msg = ' '
+    DO i = 1, 512
+      IF (cmsg(i) == C_NULL_CHAR) EXIT
+      msg(i:i) = cmsg(i)
+    ENDDO
+    CALL report_error(where, TRIM(msg))
+  END SUBROUTINE hip_check
+
+  REAL(KIND=8) FUNCTION timer()
+    INTEGER(KIND=8) :: cnt, rate
+    CALL SYSTEM_CLOCK(cnt, rate)
+    timer = REAL(cnt, 8) / REAL(rate, 8)
+  END FUNCTION timer
+
+  ! ===========================================================================
+  ! deck reader (parse.f90 semantics on one chunk)
+  ! ===========================================================================
+  !> a100 record -> blanks for control characters, left-adjusted, cut at ! and ;
+  FUNCTION clean_line(l) RESULT(c)
+    CHARACTER(LEN=*), INTENT(IN) :: l
+    CHARACTER(LEN=LINE_LEN) :: c
+    INTEGER :: i, s
+    c = l
+    DO i = 1, LEN_TRIM(c)
+      IF (IACHAR(c(i:i)) < 32 .OR. IACHAR(c(i:i)) > 128) c(i:i) = ' '
+    ENDDO
+    c = ADJUSTL(c)
+    s = SCAN(c, '!')
+    IF (s > 0) c = c(1:s-1)
+    s = SCAN(c, ';')
+    IF (s > 0) c = c(1:s-1)
+  END FUNCTION clean_line
+
+  SUBROUTINE load_deck(path)
+    CHARACTER(LEN=*), INTENT(IN) :: path
+    CHARACTER(LEN=4096) :: buf
+    CHARACTER(LEN=LINE_LEN) :: c
+    INTEGER :: u, ios, cap
+    CHARACTER(LEN=LINE_LEN), ALLOCATABLE :: tmp(:)
+    OPEN(NEWUNIT=u, FILE=path, ACTION='READ', STATUS='OLD', IOSTAT=ios)
+    IF (ios /= 0) CALL report_error('initialise', 'Error opening tea.in')
+    cap = 256
+    IF (ALLOCATED(raw_lines)) DEALLOCATE(raw_lines)
+    IF (ALLOCATED(deck_lines)) DEALLOCATE(deck_lines)
+    ALLOCATE(raw_lines(cap), deck_lines(cap))
+    n_raw = 0
+    n_deck = 0
+    DO
+      READ(u, '(a)', IOSTAT=ios) buf
+      IF (ios /= 0) EXIT
+      IF (n_raw == cap) THEN
+        ALLOCATE(tmp(2 * cap))
+        tmp(1:cap) = raw_lines
+        CALL MOVE_ALLOC(tmp, raw_lines)
+        ALLOCATE(tmp(2 * cap))
+        tmp(1:cap) = deck_lines
+        CALL MOVE_ALLOC(tmp, deck_lines)
+        cap = 2 * cap
+      ENDIF
+      n_raw = n_raw + 1
+      raw_lines(n_raw) = buf(1:LINE_LEN)
+      c = clean_line(buf(1:LINE_LEN))
+      IF (LEN_TRIM(c) > 0) THEN
+        n_deck = n_deck + 1
+        deck_lines(n_deck) = c
+      ENDIF
+    ENDDO
+    CLOSE(u)
+  END SUBROUTINE load_deck
+
+  SUBROUTINE deck_rewind()
+    deck_pos = 0
+    here = ' '
+    cur_line = ' '
+  END SUBROUTINE deck_rewind
+
+  !> next line of the *tea block (lower case, '=' and ',' as blanks);
+  !> returns .FALSE. at *endtea or the end of the deck
+  LOGICAL FUNCTION deck_getline()
+    CHARACTER(LEN=LINE_LEN) :: l
+    CHARACTER(LEN=64) :: nugget, want
+    INTEGER :: i, s
+    deck_getline = .FALSE.
+    DO WHILE (deck_pos < n_deck)
+      deck_pos = deck_pos + 1
+      l = deck_lines(deck_pos)
+      DO i = 1, LEN_TRIM(l)
+        IF (IACHAR(l(i:i)) > 64 .AND. IACHAR(l(i:i)) < 91) l(i:i) = ACHAR(IACHAR(l(i:i)) + 32)
+        IF (l(i:i) == '=' .OR. l(i:i) == ',') l(i:i) = ' '
+      ENDDO
+      IF (l(1:8) == '*select:') CYCLE
+      IF (l(1:1) == '*') THEN
+        s = SCAN(l, ' ')
+        IF (s == 0) s = LEN_TRIM(l) + 1
+        nugget = l(1:s-1)
+        IF (nugget(2:4) == 'end') THEN
+          want = '*'//TRIM(nugget(5:))
+          s = INDEX(here, '*', BACK=.TRUE.)
+          IF (s == 0) CALL report_error('parse_getline', 'Unmatched */*end pair.')
+          IF (TRIM(here(s:)) /= TRIM(want)) CALL report_error('parse_getline', 'Unmatched */*end pair.')
+          IF (TRIM(here) == '*tea') THEN
+            here = here(1:s-1)
+            RETURN
+          ENDIF
+          here = here(1:s-1)
+        ELSE
+          here = TRIM(here)//TRIM(nugget)
+        ENDIF
+        CYCLE
+      ENDIF
+      IF (TRIM(here) == '*tea' .AND. LEN_TRIM(l) > 0) THEN
+        cur_line = l
+        deck_getline = .TRUE.
+        RETURN
+      ENDIF
+    ENDDO
+  END FUNCTION deck_getline
+
+  !> next blank-separated word; with wrap an exhausted line continues on the next
+  RECURSIVE FUNCTION deck_getword(wrap) RESULT(word)
+    LOGICAL, INTENT(IN) :: wrap
+    CHARACTER(LEN=LINE_LEN) :: word
+    INTEGER :: s
+    cur_line = ADJUSTL(cur_line)
+    s = SCAN(cur_line, ' ')
+    IF (s == 0) s = LEN_TRIM(cur_line) + 1
+    word = cur_line(1:s-1)
+    IF (TRIM(word) == ACHAR(92) .OR. (LEN_TRIM(word) == 0 .AND. wrap)) THEN
+      word = ' '
+      IF (deck_getline()) word = deck_getword(wrap)
+    ELSE
+      cur_line = cur_line(s:)
+    ENDIF
+  END FUNCTION deck_getword
+
+  INTEGER FUNCTION get_ival(word)
+    CHARACTER(LEN=*), INTENT(IN) :: word
+    INTEGER :: ios
+    READ(word, '(I7)', IOSTAT=ios) get_ival
+    IF (ios /= 0) CALL report_error('parse_getival', 'Error attempting to convert to integer:'//TRIM(word))
+  END FUNCTION get_ival
+
+  REAL(KIND=8) FUNCTION get_rval(word)
+    CHARACTER(LEN=*), INTENT(IN) :: word
+    CHARACTER(LEN=LINE_LEN + 2) :: w
+    INTEGER :: ios
+    w = word
+    IF (SCAN(w, '.') == 0) w = TRIM(w)//'.0'
+    READ(w, '(E27.20)', IOSTAT=ios) get_rval
+    IF (ios /= 0) CALL report_error('parse_getrval', 'Error attempting to convert to real:'//TRIM(word))
+  END FUNCTION get_rval
+
+  !> read_input.f90: defaults, keyword switch, PPCG default, state shrink
+  SUBROUTINE read_input()
+    CHARACTER(LEN=LINE_LEN) :: word, v
+    INTEGER :: state_max, st, n
+    REAL(KIND=8) :: ddx, ddy
+
+    test_problem = 0
+    state_max = 0
+    xmin = 0.0_8; ymin = 0.0_8; xmax = 100.0_8; ymax = 100.0_8
+    x_cells = 10; y_cells = 10
+    end_time = 10.0_8; end_step = g_ibig; complete = .FALSE.
+    visit_frequency = 0; summary_frequency = 10
+    dtinit = REAL(0.1, 8)
+    max_iters = 1000; eps = REAL(1.0e-10, 8)
+    coefficient = CONDUCTIVITY
+    profiler_on = .FALSE.
+    tl_ch_cg_presteps = 25; tl_ch_cg_epslim = 1.0_8
+    tl_check_result = .FALSE.; tl_preconditioner_type = TL_PREC_NONE
+    reflective_boundary = .FALSE.
+    tl_ppcg_inner_steps = -1
+    tl_use_chebyshev = .FALSE.; tl_use_cg = .FALSE.; tl_use_ppcg = .FALSE.; tl_use_jacobi = .TRUE.
+    verbose_on = .FALSE.
+    halo_depth = 1
+
+    WRITE(g_out, *) 'Reading input file'
+    WRITE(g_out, *)
+
+    CALL deck_rewind()
+    DO WHILE (deck_getline())
+      DO
+        word = deck_getword(.FALSE.)
+        IF (LEN_TRIM(word) == 0) EXIT
+        IF (TRIM(word) == 'state') THEN
+          state_max = MAX(state_max, get_ival(deck_getword(.TRUE.)))
+          EXIT
+        ENDIF
+      ENDDO
+    ENDDO
+    number_of_states = state_max
+    IF (number_of_states < 1) CALL report_error('read_input', 'No states defined.')
+    IF (ALLOCATED(states)) DEALLOCATE(states)
+    ALLOCATE(states(number_of_states))
+
+    CALL deck_rewind()
+    DO WHILE (deck_getline())
+      DO
+        word = deck_getword(.FALSE.)
+        IF (LEN_TRIM(word) == 0) EXIT
+        SELECT CASE (TRIM(word))
+        CASE ('initial_timestep')
+          dtinit = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'initial_timestep ', dtinit
+        CASE ('end_time')
+          end_time = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'end_time', end_time
+        CASE ('end_step')
+          end_step = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'end_step', end_step
+        CASE ('xmin')
+          xmin = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'xmin', xmin
+        CASE ('xmax')
+          xmax = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'xmax', xmax
+        CASE ('ymin')
+          ymin = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'ymin', ymin
+        CASE ('ymax')
+          ymax = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'ymax', ymax
+        CASE ('x_cells')
+          x_cells = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'x_cells', x_cells
+        CASE ('y_cells')
+          y_cells = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'y_cells', y_cells
+        CASE ('visit_frequency')
+          visit_frequency = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'visit_frequency', visit_frequency
+        CASE ('summary_frequency')
+          summary_frequency = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'summary_frequency', summary_frequency
+        CASE ('tiles_per_task', 'sub_tiles_per_tile', 'tiles_per_problem')
+          ! one chunk, one tile on the GPU: the value is read and ignored
+          n = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") TRIM(word), n
+        CASE ('tl_ch_cg_presteps')
+          tl_ch_cg_presteps = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'tl_ch_cg_presteps', tl_ch_cg_presteps
+        CASE ('tl_ppcg_inner_steps')
+          tl_ppcg_inner_steps = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'tl_ppcg_inner_steps', tl_ppcg_inner_steps
+        CASE ('tl_ch_cg_epslim')
+          tl_ch_cg_epslim = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'tl_ch_cg_epslim', tl_ch_cg_epslim
+        CASE ('tl_check_result')
+          tl_check_result = .TRUE.
+        CASE ('tl_preconditioner_type')
+          DO
+            v = deck_getword(.FALSE.)
+            IF (LEN_TRIM(v) == 0) EXIT
+            SELECT CASE (TRIM(v))
+            CASE ('none')
+              tl_preconditioner_type = TL_PREC_NONE
+            CASE ('jac_diag')
+              tl_preconditioner_type = TL_PREC_JAC_DIAG
+            CASE ('jac_block')
+              tl_preconditioner_type = TL_PREC_JAC_BLOCK
+            CASE DEFAULT
+              CALL report_error('read_input', 'Invalid preconditioner type specified: '//TRIM(v))
+            END SELECT
+            WRITE(g_out, "(1x,a25,'  ',a25)") 'tl_preconditioner_type', v
+          ENDDO
+        CASE ('use_fortran_kernels')
+          CONTINUE
+        CASE ('verbose_on')
+          verbose_on = .TRUE.
+        CASE ('tl_use_jacobi')
+          tl_use_chebyshev = .FALSE.; tl_use_cg = .FALSE.; tl_use_ppcg = .FALSE.; tl_use_jacobi = .TRUE.
+        CASE ('tl_use_cg')
+          tl_use_chebyshev = .FALSE.; tl_use_cg = .TRUE.; tl_use_ppcg = .FALSE.; tl_use_jacobi = .FALSE.
+        CASE ('tl_use_ppcg')
+          tl_use_chebyshev = .FALSE.; tl_use_cg = .FALSE.; tl_use_ppcg = .TRUE.; tl_use_jacobi = .FALSE.
+        CASE ('tl_use_chebyshev')
+          tl_use_chebyshev = .TRUE.; tl_use_cg = .FALSE.; tl_use_ppcg = .FALSE.; tl_use_jacobi = .FALSE.
+        CASE ('reflective_boundary')
+          reflective_boundary = .TRUE.
+        CASE ('profiler_on')
+          profiler_on = .TRUE.
+          WRITE(g_out, "(1x,a25)") 'Profiler on'
+        CASE ('halo_depth')
+          halo_depth = get_ival(deck_getword(.TRUE.))
+          IF (halo_depth < 1) CALL report_error('read_input', 'Invalid halo exchange depth specified')
+          WRITE(g_out, "(1x,a25,i12)") 'halo_depth', halo_depth
+        CASE ('tl_max_iters')
+          max_iters = get_ival(deck_getword(.TRUE.))
+        CASE ('tl_eps')
+          eps = get_rval(deck_getword(.TRUE.))
+        CASE ('tl_coefficient_density')
+          coefficient = CONDUCTIVITY
+          WRITE(g_out, "(1x,a29)") 'Diffusion coefficient density'
+        CASE ('tl_coefficient_inverse_density')
+          coefficient = RECIP_CONDUCTIVITY
+          WRITE(g_out, "(1x,a40)") 'Diffusion coefficient reciprocal density'
+        CASE ('test_problem')
+          test_problem = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'test_problem', test_problem
+        CASE ('state')
+          st = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, *) 'Reading specification for state ', st
+          IF (st < 1 .OR. st > number_of_states) CALL report_error('read_input', 'Invalid state number.')
+          IF (states(st)%defined) CALL report_error('read_input', 'State defined twice.')
+          WRITE(g_out, *)
+          states(st)%defined = .TRUE.
+          DO
+            word = deck_getword(.FALSE.)
+            IF (LEN_TRIM(word) == 0) EXIT
+            SELECT CASE (TRIM(word))
+            CASE ('xmin')
+              states(st)%xmin = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state xmin ', states(st)%xmin
+            CASE ('ymin')
+              states(st)%ymin = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state ymin ', states(st)%ymin
+            CASE ('xmax')
+              states(st)%xmax = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state xmax ', states(st)%xmax
+            CASE ('ymax')
+              states(st)%ymax = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state ymax ', states(st)%ymax
+            CASE ('radius')
+              states(st)%radius = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state radius ', states(st)%radius
+            CASE ('density')
+              states(st)%density = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state density ', states(st)%density
+            CASE ('energy')
+              states(st)%energy = get_rval(deck_getword(.TRUE.))
+              WRITE(g_out, "(1x,a25,e12.4)") 'state energy ', states(st)%energy
+            CASE ('geometry')
+              v = deck_getword(.TRUE.)
+              SELECT CASE (TRIM(v))
+              CASE ('rectangle')
+                states(st)%geometry = g_rect
+                WRITE(g_out, "(1x,a26)") 'state geometry rectangular'
+              CASE ('circle')
+                states(st)%geometry = g_circ
+                WRITE(g_out, "(1x,a25)") 'state geometry circular'
+              CASE ('point')
+                states(st)%geometry = g_point
+                WRITE(g_out, "(1x,a25)") 'state geometry point'
+              END SELECT
+            END SELECT
+          ENDDO
+          WRITE(g_out, *)
+        END SELECT
+      ENDDO
+    ENDDO
+
+    IF (tl_ppcg_inner_steps == -1 .AND. tl_use_ppcg) THEN
+      tl_ppcg_inner_steps = 4 * INT(SQRT(SQRT(REAL(x_cells * y_cells))))
+      WRITE(g_out, "(1x,a25,i12)") 'tl_ppcg_inner_steps', tl_ppcg_inner_steps
+    ENDIF
+    IF (halo_depth > 1 .AND. tl_preconditioner_type == TL_PREC_JAC_BLOCK) &
+      CALL report_error('read_input', 'Unable to use nonstandard halo depth with block jacobi preconditioner')
+
+    WRITE(g_out, "(1x,a25,i12)") 'tiles per task ', 1
+    WRITE(g_out, *)
+    WRITE(g_out, "(1x,a)") 'Using HIP kernels (gfx950)'
+    WRITE(g_out, *)
+    WRITE(g_out, *) 'Input read finished.'
+    WRITE(g_out, *)
+    FLUSH(g_out)
+
+    ! state extents move by a hundredth of a cell (read_input.f90:346-359)
+    ddx = (xmax - xmin) / REAL(x_cells)
+    ddy = (ymax - ymin) / REAL(y_cells)
+    DO n = 2, number_of_states
+      states(n)%xmin = states(n)%xmin + (ddx / 100.0_8)
+      states(n)%ymin = states(n)%ymin + (ddy / 100.0_8)
+      states(n)%xmax = states(n)%xmax - (ddx / 100.0_8)
+      states(n)%ymax = states(n)%ymax - (ddy / 100.0_8)
+    ENDDO
+  END SUBROUTINE read_input
+
+  ! ===========================================================================
+  ! setup: geometry, state painting, upload (start.f90 on one chunk)
+  ! ===========================================================================
+  SUBROUTINE generate_and_upload()
+    REAL(KIND=8), ALLOCATABLE :: vx(:), vy(:), cx(:), cy(:), density(:,:), energy0(:,:), u(:,:), vol(:,:)
+    INTEGER :: j, k, s, h
+    REAL(KIND=8) :: rad, xc, yc
+    LOGICAL :: hit
+    h = halo_depth
+    dx = (xmax - xmin) / REAL(x_cells)
+    dy = (ymax - ymin) / REAL(y_cells)
+    ALLOCATE(vx(-1:x_cells + 3), vy(-1:y_cells + 3), cx(-1:x_cells + 2), cy(-1:y_cells + 2))
+    DO j = -1, x_cells + 3
+      vx(j) = xmin + dx * REAL(j - 1)
+    ENDDO
+    DO k = -1, y_cells + 3
+      vy(k) = ymin + dy * REAL(k - 1)
+    ENDDO
+    DO j = -1, x_cells + 2
+      cx(j) = 0.5 * (vx(j) + vx(j + 1))
+    ENDDO
+    DO k = -1, y_cells + 2
+      cy(k) = 0.5 * (vy(k) + vy(k + 1))
+    ENDDO
+    ALLOCATE(vol(-1:x_cells + 2, -1:y_cells + 2))
+    vol = dx * dy
+    CALL hip_check(tl_upload(TLF_VOLUME, vol, 2_c_int), 'start')
+    DEALLOCATE(vol)
+
+    ALLOCATE(density(1 - h:x_cells + h, 1 - h:y_cells + h), energy0(1 - h:x_cells + h, 1 - h:y_cells + h))
+    energy0 = states(1)%energy
+    density = states(1)%density
+    DO s = 2, number_of_states
+      xc = states(s)%xmin
+      yc = states(s)%ymin
+      DO k = 1 - h, y_cells + h
+        DO j = 1 - h, x_cells + h
+          hit = .FALSE.
+          SELECT CASE (states(s)%geometry)
+          CASE (g_rect)
+            IF (j >= 1 .AND. j <= x_cells .AND. k >= 1 .AND. k <= y_cells) THEN
+              hit = vx(j + 1) >= states(s)%xmin .AND. vx(j) < states(s)%xmax .AND. &
+                    vy(k + 1) >= states(s)%ymin .AND. vy(k) < states(s)%ymax
+            ENDIF
+          CASE (g_circ)
+            rad = SQRT((cx(MIN(MAX(j, -1), x_cells + 2)) - xc) * (cx(MIN(MAX(j, -1), x_cells + 2)) - xc) + &
+                       (cy(MIN(MAX(k, -1), y_cells + 2)) - yc) * (cy(MIN(MAX(k, -1), y_cells + 2)) - yc))
+            hit = rad <= states(s)%radius
+          CASE (g_point)
+            hit = vx(MIN(MAX(j, -1), x_cells + 3)) == xc .AND. vy(MIN(MAX(k, -1), y_cells + 3)) == yc
+          END SELECT
+          IF (hit) THEN
+            energy0(j, k) = states(s)%energy
+            density(j, k) = states(s)%density
+          ENDIF
+        ENDDO
+      ENDDO
+    ENDDO
+    ALLOCATE(u(1 - h:x_cells + h, 1 - h:y_cells + h))
+    u = energy0 * density
+    CALL hip_check(tl_upload(TLF_DENSITY, density, INT(h, c_int)), 'start')
+    CALL hip_check(tl_upload(TLF_ENERGY0, energy0, INT(h, c_int)), 'start')
+    CALL hip_check(tl_upload(TLF_U, u, INT(h, c_int)), 'start')
+    DEALLOCATE(vx, vy, cx, cy, density, energy0, u)
+  END SUBROUTINE generate_and_upload
+
+  ! ===========================================================================
+  ! drivers of the reference wrappers (one chunk: the C calls do the work)
+  ! ===========================================================================
+  SUBROUTINE update_halo(fields, depth)
+    INTEGER, INTENT(IN) :: fields(NUM_FIELDS), depth
+    INTEGER(c_int) :: f(NUM_FIELDS)
+    REAL(KIND=8) :: t0
+    IF (.NOT. reflective_boundary) RETURN
+    IF (profiler_on) t0 = timer()
+    f = INT(fields, c_int)
+    CALL hip_check(tl_update_halo(f, INT(depth, c_int)), 'update_halo')
+    IF (profiler_on) profiler%halo_update = profiler%halo_update + (timer() - t0)
+  END SUBROUTINE update_halo
+
+  SUBROUTINE field_summary()
+    REAL(KIND=8) :: vol, mass, ie, temp, qa_diff, t0
+    WRITE(g_out, *)
+    WRITE(g_out, *) 'Time ', time
+    WRITE(g_out, '(a13,5a26)') '           ', 'Volume', 'Mass', 'Density', 'Energy', 'U'
+    IF (profiler_on) t0 = timer()
+    CALL hip_check(tl_field_summary(vol, mass, ie, temp), 'field_summary')
+    IF (profiler_on) profiler%summary = profiler%summary + (timer() - t0)
+    WRITE(g_out, '(a6,i7,5e26.17)') ' step:', step, vol, mass, mass / vol, ie, temp
+    WRITE(g_out, *)
+    FLUSH(g_out)
+    last_summary = [vol, mass, mass / vol, ie, temp]
+    last_summary_step = step
+    IF (complete .AND. test_problem >= 1) THEN
+      ! expected final temperatures of field_summary.f90:110-115
+      qa_diff = -1.0_8
+      IF (test_problem == 1) qa_diff = ABS((100.0_8 * (temp / 157.55084183279294_8)) - 100.0_8)
+      IF (test_problem == 2) qa_diff = ABS((100.0_8 * (temp / 106.27221178646569_8)) - 100.0_8)
+      IF (test_problem == 3) qa_diff = ABS((100.0_8 * (temp / 99.955877498324000_8)) - 100.0_8)
+      IF (test_problem == 4) qa_diff = ABS((100.0_8 * (temp / 97.277332050749976_8)) - 100.0_8)
+      IF (test_problem == 5) qa_diff = ABS((100.0_8 * (temp / 95.462351583362249_8)) - 100.0_8)
+      IF (test_problem == 6) qa_diff = ABS((100.0_8 * (temp / 95.174738768320850_8)) - 100.0_8)
+      IF (qa_diff >= 0.0_8) THEN
+        WRITE(*, '(a,i4,a,e16.7,a)') "Test problem", test_problem, " is within", qa_diff, "% of the expected solution"
+        WRITE(g_out, '(a,i4,a,e16.7,a)') "Test problem", test_problem, " is within", qa_diff, "% of the expected solution"
+        IF (qa_diff < 0.001) THEN
+          WRITE(*, *) "This test is considered PASSED"
+          WRITE(g_out, *) "This test is considered PASSED"
+        ELSE
+          WRITE(*, *) "This test is considered NOT PASSED"
+          WRITE(g_out, *) "This is test is considered NOT PASSED"
+        ENDIF
+      ENDIF
+    ENDIF
+  END SUBROUTINE field_summary
+
+  SUBROUTINE timestep()
+    REAL(KIND=8) :: t0
+    IF (profiler_on) t0 = timer()
+    dt = dtinit                                    ! calc_dt.f90:34, tea_min on one chunk
+    IF (profiler_on) profiler%timestep = profiler%timestep + (timer() - t0)
+    WRITE(g_out, "(' Step ', i7,' time ', f11.7,' timestep  ',1pe9.2,i8)") step, time, dt
+    WRITE(0, "(' Step ', i7,' time ', f11.7,' timestep  ',1pe9.2,i8)") step, time, dt
+  END SUBROUTINE timestep
+
+  ! ---- host scalar routines of tea_leaf_cheby.f90 ---------------------------
+  !> Eigenvalues of the symmetric tridiagonal (d, e) by the implicit QL
+  !> method with Wilkinson-style shifts; e(2:n) holds the sub-diagonal on
+  !> entry.  info = 1 after 30 sweeps without convergence of an eigenvalue.
+  SUBROUTINE ql_eigenvalues(d, e, n, info)
+    INTEGER, INTENT(IN) :: n
+    REAL(KIND=8), INTENT(INOUT) :: d(n), e(n)
+    INTEGER, INTENT(OUT) :: info
+    INTEGER :: l, m, i, sweeps
+    REAL(KIND=8) :: g, r, s, c, p, f, b, tol
+    LOGICAL :: underflow
+    info = 0
+    IF (n < 1) RETURN
+    e(1:n-1) = e(2:n)
+    e(n) = 0.0_8
+    DO l = 1, n
+      sweeps = 0
+      DO
+        ! find the first negligible sub-diagonal at or after l
+        m = l
+        DO WHILE (m < n)
+          tol = ABS(d(m)) + ABS(d(m + 1))
+          IF (ABS(e(m)) + tol == tol) EXIT
+          m = m + 1
+        ENDDO
+        IF (m == l) EXIT
+        IF (sweeps == 30) THEN
+          info = 1
+          RETURN
+        ENDIF
+        sweeps = sweeps + 1
+        g = (d(l + 1) - d(l)) / (2.0_8 * e(l))
+        r = SQRT(g * g + 1.0_8)
+        g = d(m) - d(l) + e(l) / (g + SIGN(r, g))
+        s = 1.0_8
+        c = 1.0_8
+        p = 0.0_8
+        underflow = .FALSE.
+        DO i = m - 1, l, -1
+          f = s * e(i)
+          b = c * e(i)
+          r = SQRT(f * f + g * g)
+          e(i + 1) = r
+          IF (r == 0.0_8) THEN
+            d(i + 1) = d(i + 1) - p
+            e(m) = 0.0_8
+            underflow = .TRUE.
+            EXIT
+          ENDIF
+          s = f / r
+          c = g / r
+          g = d(i + 1) - p
+          r = (d(i) - g) * s + 2.0_8 * c * b
+          p = s * r
+          d(i + 1) = g + p
+          g = c * r - b
+        ENDDO
+        IF (underflow) CYCLE
+        d(l) = d(l) - p
+        e(l) = g
+        e(m) = 0.0_8
+      ENDDO
+    ENDDO
+  END SUBROUTINE ql_eigenvalues
+
+  !> Lanczos tridiagonal from the CG coefficients and its extreme eigenvalues
+  !> (tea_leaf_cheby.f90:142-176).
+  SUBROUTINE calc_eigenvalues(al, be, nsteps, emin, emax, info)
+    INTEGER, INTENT(IN) :: nsteps
+    REAL(KIND=8), INTENT(IN) :: al(0:), be(0:)
+    REAL(KIND=8), INTENT(OUT) :: emin, emax
+    INTEGER, INTENT(OUT) :: info
+    REAL(KIND=8), ALLOCATABLE :: d(:), e(:)
+    INTEGER :: i
+    ALLOCATE(d(MAX(nsteps, 1)), e(MAX(nsteps, 1)))
+    d = 0.0_8
+    e = 0.0_8
+    DO i = 1, nsteps
+      d(i) = 1.0_8 / al(i)
+      IF (i > 1) d(i) = d(i) + be(i - 1) / al(i - 1)
+      IF (i < nsteps) e(i + 1) = SQRT(be(i)) / al(i)
+    ENDDO
+    CALL ql_eigenvalues(d, e, nsteps, info)
+    emin = 0.0_8
+    emax = 0.0_8
+    IF (info /= 0) RETURN
+    emin = MINVAL(d(1:nsteps))
+    emax = MAXVAL(d(1:nsteps))
+    IF (emin < 0.0_8 .OR. emax < 0.0_8) info = 1
+  END SUBROUTINE calc_eigenvalues
+
+  !> Chebyshev recurrence coefficients (tea_leaf_cheby.f90:178-205)
+  SUBROUTINE calc_ch_coefs(ca, cb, emin, emax, th, n)
+    INTEGER, INTENT(IN) :: n
+    REAL(KIND=8), INTENT(OUT) :: ca(:), cb(:), th
+    REAL(KIND=8), INTENT(IN) :: emin, emax
+    REAL(KIND=8) :: delta, sigma, rho_old, rho_new
+    INTEGER :: i
+    th = (emax + emin) / 2
+    delta = (emax - emin) / 2
+    sigma = th / delta
+    rho_old = 1.0_8 / sigma
+    DO i = 1, n
+      rho_new = 1.0_8 / (2.0_8 * sigma - rho_old)
+      ca(i) = rho_new * rho_old
+      cb(i) = 2.0_8 * rho_new / delta
+      rho_old = rho_new
+    ENDDO
+  END SUBROUTINE calc_ch_coefs
+
+  ! ===========================================================================
+  ! tea_leaf(): one implicit solve (tea_solve.f90:37-503).  The iteration loop
+  ! runs in segments: the fused device loops advance many iterations per call
+  ! and stop exactly where the reference's loop body would branch (exit test,
+  ! CG->Chebyshev/PPCG switch, max_iters).
+  ! ===========================================================================
+  SUBROUTINE tea_leaf()
+    INTEGER :: n, k, m, est_itc, cheby_calc_steps, max_cheby_iters, ppcg_inner_iters, info, nc
+    LOGICAL :: ch_switch_check, exited, is_check
+    REAL(KIND=8) :: old_error, error, exact_error, initial_residual, rro, bb, e_sim, norm
+    REAL(KIND=8) :: it_alpha, gamm, t_total, t_init, t_solve, t_reset
+    INTEGER :: fields(NUM_FIELDS)
+    INTEGER(c_int) :: zb(4), inner_c
+    REAL(KIND=8), ALLOCATABLE :: cg_alphas(:), cg_betas(:), ch_alphas(:), ch_betas(:)
+    TYPE(tl_loop_state) :: st
+    REAL(c_double) :: rro_c
+
+    nc = MAX(max_iters, tl_ppcg_inner_steps, 1) + 3
+    ALLOCATE(cg_alphas(0:max_iters + 2), cg_betas(0:max_iters + 2), ch_alphas(nc), ch_betas(nc))
+    cg_alphas = 0.0_8
+    cg_betas = 0.0_8
+    ch_alphas = 0.0_8
+    ch_betas = 0.0_8
+    ppcg_inner_iters = 0
+    ch_switch_check = .FALSE.
+    cheby_calc_steps = 0
+    est_itc = 0
+    last_switch = 0
+    last_est_itc = 0
+    error = 0.0_8
+
+    IF (coefficient /= RECIP_CONDUCTIVITY .AND. coefficient /= CONDUCTIVITY) &
+      CALL report_error('tea_leaf', 'unknown coefficient option')
+    CALL hip_check(tl_set_ppcg_active(0_c_int), 'tea_leaf')
+
+    t_total = timer()
+    t_init = t_total
+
+    fields = 0
+    fields(FIELD_ENERGY1) = 1
+    fields(FIELD_DENSITY) = 1
+    CALL update_halo(fields, halo_depth)
+
+    ! tea_leaf_init_common (tea_leaf_common.f90:12-61): one chunk => all faces external
+    zb = 1
+    CALL hip_check(tl_common_init(dt / (dx * dx), dt / (dy * dy), zb), 'tea_leaf_init_common')
+
+    fields = 0
+    fields(FIELD_U) = 1
+    CALL update_halo(fields, 1)
+
+    CALL hip_check(tl_calc_residual(), 'tea_leaf_calc_residual')
+    CALL hip_check(tl_calc_2norm(1_c_int, initial_residual), 'tea_leaf_calc_2norm')
+
+    old_error = initial_residual
+    initial_residual = SQRT(ABS(initial_residual))
+    IF (verbose_on) WRITE(g_out, *) "Initial residual ", initial_residual
+
+    IF (tl_use_cg .OR. tl_use_chebyshev .OR. tl_use_ppcg) THEN
+      CALL hip_check(tl_cg_init(rro), 'tea_leaf_cg_init')
+      fields = 0
+      fields(FIELD_U) = 1
+      fields(FIELD_P) = 1
+      CALL update_halo(fields, 1)
+      fields = 0
+      fields(FIELD_P) = 1
+    ELSE IF (tl_use_jacobi) THEN
+      fields = 0
+      fields(FIELD_U) = 1
+    ENDIF
+
+    IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init)
+    t_solve = timer()
+
+    n = 1
+    exited = .FALSE.
+    outer: DO WHILE (n <= max_iters)
+
+      IF (.NOT. ch_switch_check) THEN             ! tea_solve.f90:160-174
+        IF (cheby_calc_steps > 0) THEN
+          ch_switch_check = .TRUE.
+        ELSE IF ((.NOT. first) .AND. tl_use_ppcg .AND. n > 1) THEN
+          ch_switch_check = .TRUE.
+        ELSE IF ((ABS(old_error) <= tl_ch_cg_epslim) .AND. (n >= tl_ch_cg_presteps)) THEN
+          ch_switch_check = .TRUE.
+        ENDIF
+      ENDIF
+
+      IF ((tl_use_chebyshev .OR. tl_use_ppcg) .AND. ch_switch_check) THEN
+        IF (cheby_calc_steps == 0) THEN          ! :183-253
+          max_cheby_iters = max_iters - n + 2
+          IF (first) THEN
+            CALL calc_eigenvalues(cg_alphas, cg_betas, n - 1, eigmin, eigmax, info)
+            first = .FALSE.
+            IF (info /= 0) CALL report_error('tea_leaf', 'Error in calculating eigenvalues')
+            eigmin = eigmin * 0.95_8
+            eigmax = eigmax * 1.05_8
+          ENDIF
+          IF (tl_use_chebyshev) THEN
+            IF (max_cheby_iters + 1 > SIZE(ch_alphas)) THEN
+              DEALLOCATE(ch_alphas, ch_betas)
+              ALLOCATE(ch_alphas(max_cheby_iters + 1), ch_betas(max_cheby_iters + 1))
+            ENDIF
+            CALL calc_ch_coefs(ch_alphas, ch_betas, eigmin, eigmax, theta, max_cheby_iters)
+            fields = 0
+            fields(FIELD_U) = 1
+          ELSE IF (tl_use_ppcg) THEN
+            CALL calc_ch_coefs(ch_alphas, ch_betas, eigmin, eigmax, theta, tl_ppcg_inner_steps)
+            CALL hip_check(tl_set_ppcg_active(1_c_int), 'tea_leaf')
+          ENDIF
+          cn = eigmax / eigmin
+          last_switch = n
+          WRITE(g_out, '(a,i3,a,e15.7)') "Switching after ", n, " CG its, error ", rro
+          WRITE(g_out, "('Eigen min',e14.6,' Eigen max',e14.6,' Condition number',f14.6,' Error',e14.6)") &
+            eigmin, eigmax, cn, old_error
+          WRITE(0, '(a,i3,a,e15.7)') "Switching after ", n, " CG its, error ", rro
+          WRITE(0, "('Eigen min',e14.6,' Eigen max',e14.6,' Condition number',f14.6,' Error',e14.6)") &
+            eigmin, eigmax, cn, old_error
+          ! re-initialise CG with the polynomial preconditioner applied
+          CALL hip_check(tl_ppcg_init(2_c_int, rro_c), 'tea_leaf_ppcg_init')
+          rro = rro_c
+          CALL run_ppcg_inner_steps(ch_alphas, ch_betas, tl_ppcg_inner_steps)
+          ppcg_inner_iters = ppcg_inner_iters + tl_ppcg_inner_steps
+          CALL hip_check(tl_ppcg_init(3_c_int, rro_c), 'tea_leaf_ppcg_init')
+          rro = rro_c
+          fields = 0
+          fields(FIELD_P) = 1
+          CALL update_halo(fields, 1)
+        ENDIF
+
+        IF (tl_use_chebyshev) THEN
+          IF (cheby_calc_steps == 0) THEN
+            ! tea_leaf_cheby_first_step (tea_leaf_cheby.f90:208-252); its error
+            ! argument is old_error at the call site (tea_solve.f90:261)
+            CALL hip_check(tl_calc_2norm(0_c_int, bb), 'tea_leaf_calc_2norm')
+            CALL hip_check(tl_cheby_init(theta), 'tea_leaf_cheby_init')
+            CALL update_halo(fields, 1)
+            CALL hip_check(tl_cheby_iterate(ch_alphas(1), ch_betas(1)), 'tea_leaf_cheby_iterate')
+            CALL hip_check(tl_calc_2norm(1_c_int, old_error), 'tea_leaf_calc_2norm')
+            it_alpha = eps / 2.0_8 * SQRT(bb / old_error)
+            gamm = (SQRT(cn) - 1.0_8) / (SQRT(cn) + 1.0_8)
+            est_itc = NINT(LOG(it_alpha) / (LOG(gamm)))
+            last_est_itc = est_itc
+            WRITE(g_out, '(a11)') "est itc"
+            WRITE(g_out, '(11i11)') est_itc
+            WRITE(0, '(a11)') "est itc"
+            WRITE(0, '(11i11)') est_itc
+            cheby_calc_steps = 2                   ! 1, then :333
+            CALL update_halo(fields, 1)
+            error = SQRT(ABS(error))
+            IF (verbose_on) WRITE(g_out, *) "Residual ", error
+            IF (ABS(error) < eps * initial_residual) THEN
+              exited = .TRUE.
+              EXIT outer
+            ENDIF
+            old_error = error
+            n = n + 1
+          ELSE
+            ! queue iterations n..m without a host round trip: m is the next
+            ! residual check (:273), the next exit of the stale error (:402-410)
+            ! or max_iters
+            e_sim = error
+            m = n
+            DO
+              IF ((m >= est_itc .AND. MOD(m, 10) == 0) .OR. m == max_iters) EXIT
+              e_sim = SQRT(ABS(e_sim))
+              IF (ABS(e_sim) < eps * initial_residual) EXIT
+              m = m + 1
+            ENDDO
+            is_check = (m >= est_itc .AND. MOD(m, 10) == 0)
+            norm = 0.0_8
+            CALL hip_check(tl_cheby_run(ch_alphas, ch_betas, INT(cheby_calc_steps, c_int), &
+                           INT(m - n + 1, c_int), MERGE(1_c_int, 0_c_int, is_check), norm), &
+                           'tea_leaf_cheby_iterate')
+            IF (reflective_boundary) CALL update_halo(fields, 1)
+            DO k = n, m
+              IF (k == m .AND. is_check) error = norm
+              cheby_calc_steps = cheby_calc_steps + 1
+              error = SQRT(ABS(error))
+              IF (ABS(error) < eps * initial_residual) THEN
+                n = k
+                exited = .TRUE.
+                EXIT outer
+              ENDIF
+              old_error = error
+            ENDDO
+            n = m + 1
+          ENDIF
+        ELSE IF (tl_use_ppcg) THEN
+          st%n_first = n
+          st%n_last = max_iters
+          st%eps_abs = eps * initial_residual
+          st%check_switch = 0
+          st%epslim = tl_ch_cg_epslim
+          st%presteps = tl_ch_cg_presteps
+          st%rro = rro
+          st%old_error = old_error
+          inner_c = INT(ppcg_inner_iters, c_int)
+          CALL hip_check(tl_ppcg_loop(st, ch_alphas, ch_betas, INT(tl_ppcg_inner_steps, c_int), &
+                                      theta, inner_c), 'tea_leaf_ppcg')
+          ppcg_inner_iters = inner_c
+          cheby_calc_steps = cheby_calc_steps + (st%n_end - n + 1)
+          rro = st%rro
+          error = st%error
+          IF (st%reason == TL_LOOP_CONVERGED) THEN
+            n = st%n_end
+            exited = .TRUE.
+          ELSE
+            n = max_iters + 1
+          ENDIF
+          EXIT outer
+        ENDIF
+
+      ELSE IF (tl_use_cg .OR. tl_use_chebyshev .OR. tl_use_ppcg) THEN
+        fields(FIELD_P) = 1
+        st%n_first = n
+        st%n_last = max_iters
+        ! on later steps PPCG switches at the top of iteration 2 (:164)
+        IF ((.NOT. first) .AND. tl_use_ppcg) st%n_last = MIN(max_iters, MAX(n, 1))
+        st%eps_abs = eps * initial_residual
+        st%check_switch = MERGE(1, 0, tl_use_chebyshev .OR. tl_use_ppcg)
+        st%epslim = tl_ch_cg_epslim
+        st%presteps = tl_ch_cg_presteps
+        st%rro = rro
+        st%old_error = old_error
+        CALL hip_check(tl_cg_loop(st, cg_alphas, cg_betas), 'tea_leaf_cg')
+        rro = st%rro
+        error = st%error
+        SELECT CASE (st%reason)
+        CASE (TL_LOOP_CONVERGED)
+          n = st%n_end
+          exited = .TRUE.
+          EXIT outer
+        CASE (TL_LOOP_SWITCH)
+          n = st%n_end
+          old_error = st%old_error
+        CASE DEFAULT
+          old_error = error
+          n = st%n_end + 1
+        END SELECT
+
+      ELSE IF (tl_use_jacobi) THEN
+        st%n_first = n
+        st%n_last = max_iters
+        st%eps_abs = eps * initial_residual
+        st%check_switch = 0
+        CALL hip_check(tl_jacobi_loop(st), 'tea_leaf_jacobi')
+        error = st%error
+        IF (st%reason == TL_LOOP_CONVERGED) THEN
+          n = st%n_end
+          exited = .TRUE.
+        ELSE
+          n = max_iters + 1
+        ENDIF
+        EXIT outer
+      ENDIF
+    ENDDO outer
+    IF (.NOT. exited) n = max_iters + 1
+
+    IF (tl_check_result) THEN                        ! :416-431
+      fields = 0
+      fields(FIELD_U) = 1
+      CALL update_halo(fields, 1)
+      CALL hip_check(tl_calc_residual(), 'tea_leaf_calc_residual')
+      CALL hip_check(tl_calc_2norm(1_c_int, exact_error), 'tea_leaf_calc_2norm')
+      exact_error = SQRT(exact_error)
+    ENDIF
+
+    IF (profiler_on) profiler%tea_solve = profiler%tea_solve + (timer() - t_solve)
+
+    WRITE(g_out, "('Conduction error ',e14.7)") error / initial_residual
+    WRITE(0, "('Conduction error ',e14.7)") error / initial_residual
+    IF (tl_check_result) THEN
+      WRITE(0, "('EXACT error calculated as', e14.7)") exact_error / initial_residual
+      WRITE(g_out, "('EXACT error calculated as', e14.7)") exact_error / initial_residual
+      last_exact_error = exact_error / initial_residual
+    ENDIF
+    WRITE(g_out, "('Iteration count ',i8)") n - 1
+    WRITE(0, "('Iteration count ', i8)") n - 1
+    IF (tl_use_ppcg) THEN
+      WRITE(g_out, "('PPCG Iteration count', i8, ' (Total ',i8,')')") ppcg_inner_iters, ppcg_inner_iters + n - 1
+      WRITE(0, "('PPCG Iteration count', i8, ' (Total ',i8,')')") ppcg_inner_iters, ppcg_inner_iters + n - 1
+    ENDIF
+    last_iterations = n - 1
+    last_ppcg_inner = ppcg_inner_iters
+    last_cond_error = error / initial_residual
+
+    t_reset = timer()
+    CALL hip_check(tl_finalise(), 'tea_leaf_finalise')
+    fields = 0
+    fields(FIELD_ENERGY1) = 1
+    CALL update_halo(fields, 1)
+    IF (profiler_on) profiler%tea_reset = profiler%tea_reset + (timer() - t_reset)
+
+    CALL hip_check(tl_synchronize(), 'tea_leaf')
+    t_total = timer() - t_total
+    last_solve_time = t_total
+    IF (profiler_on) THEN
+      WRITE(0, "(a16,f16.10,a7,i7,a16,f16.10)") "Solve Time", t_total, "Its", n, "Time Per It", t_total / n
+      WRITE(g_out, "(a16,f16.10,a7,i7,a16,f16.10)") "Solve Time", t_total, "Its", n, "Time Per It", t_total / n
+    ENDIF
+    DEALLOCATE(cg_alphas, cg_betas, ch_alphas, ch_betas)
+  END SUBROUTINE tea_leaf
+
+  !> tea_leaf_ppcg.f90:211-431 on one chunk
+  SUBROUTINE run_ppcg_inner_steps(ca, cb, inner)
+    REAL(KIND=8), INTENT(IN) :: ca(:), cb(:)
+    INTEGER, INTENT(IN) :: inner
+    INTEGER :: fields(NUM_FIELDS)
+    IF (inner < 0) RETURN
+    CALL hip_check(tl_ppcg_init_sd(theta), 'tea_leaf_ppcg_init_sd')
+    fields = 0
+    fields(FIELD_SD) = 1
+    fields(FIELD_R) = 1
+    CALL update_halo(fields, halo_depth)
+    IF (inner > 0) CALL hip_check(tl_ppcg_inner(ca, cb, 1_c_int, INT(inner, c_int)), &
+                                  'tea_leaf_run_ppcg_inner_steps')
+    CALL hip_check(tl_ppcg_update_z(), 'tea_leaf_ppcg_update_z')
+  END SUBROUTINE run_ppcg_inner_steps
+
+  ! ===========================================================================
+  ! program flow: initialise (initialise.f90 + start.f90), one diffuse step,
+  ! the completion report (diffuse.f90)
+  ! ===========================================================================
+  SUBROUTINE initialise(deck_path, out_path, device)
+    CHARACTER(LEN=*), INTENT(IN) :: deck_path, out_path
+    INTEGER, INTENT(IN) :: device
+    INTEGER :: ios, i
+    INTEGER :: fields(NUM_FIELDS)
+
+    IF (LEN_TRIM(out_path) > 0) THEN
+      OPEN(NEWUNIT=g_out, FILE=TRIM(out_path), ACTION='WRITE', STATUS='REPLACE', IOSTAT=ios)
+      IF (ios /= 0) CALL report_error('initialise', 'Error opening tea.out file.')
+      out_is_file = .TRUE.
+    ELSE
+      g_out = 6
+      out_is_file = .FALSE.
+    ENDIF
+    WRITE(g_out, *)
+    WRITE(g_out, '(a15,f8.3)') 'Tea Version ', g_version
+    WRITE(g_out, '(a18)') 'HIP Version'
+    WRITE(g_out, '(a14,i6)') 'Task Count ', 1
+    WRITE(g_out, *)
+    WRITE(g_out, *) 'Tea will run from the following input:-'
+    WRITE(g_out, *)
+
+    CALL load_deck(deck_path)
+    DO i = 1, n_raw
+      WRITE(g_out, '(a100)') raw_lines(i)
+    ENDDO
+    WRITE(g_out, *)
+    WRITE(g_out, *) 'Initialising and generating'
+    WRITE(g_out, *)
+
+    CALL read_input()
+
+    step = 0
+    time = 0.0_8
+    dt = dtinit
+    first = .TRUE.
+    eigmin = 0.0_8
+    eigmax = 0.0_8
+    theta = 0.0_8
+    cn = 0.0_8
+    profiler = profiler_type()
+
+    WRITE(g_out, *) 'Setting up initial geometry'
+    WRITE(g_out, *)
+    WRITE(g_out, *) "Tile size ", x_cells, " by ", y_cells, " cells"
+    CALL hip_check(tl_init(INT(x_cells, c_int), INT(y_cells, c_int), INT(halo_depth, c_int), &
+                           INT(device, c_int)), 'start')
+    host_ready = .TRUE.
+    CALL hip_check(tl_set_options(INT(coefficient, c_int), INT(tl_preconditioner_type, c_int), &
+                                  MERGE(1_c_int, 0_c_int, reflective_boundary)), 'start')
+    WRITE(g_out, *) 'Generating chunk '
+    CALL generate_and_upload()
+
+    fields = 0
+    fields(FIELD_DENSITY) = 1
+    fields(FIELD_ENERGY0) = 1
+    fields(FIELD_ENERGY1) = 1
+    CALL update_halo(fields, halo_depth)
+    WRITE(g_out, *)
+    WRITE(g_out, *) 'Problem initialised and generated'
+    CALL hip_check(tl_set_field(), 'set_field')
+    CALL field_summary()
+    WRITE(g_out, *) 'Starting the calculation'
+    complete = .FALSE.
+    timerstart = timer()
+  END SUBROUTINE initialise
+
+  !> One pass of the diffuse() loop body (diffuse.f90:45-105); returns
+  !> .TRUE. once the run is complete.
+  LOGICAL FUNCTION diffuse_step()
+    REAL(KIND=8) :: step_time, wall_clock, step_clock, cells
+    step_time = timer()
+    step = step + 1
+    CALL timestep()
+    CALL tea_leaf()
+    time = time + dt
+    IF (summary_frequency /= 0) THEN
+      IF (MOD(step, summary_frequency) == 0) CALL field_summary()
+    ENDIF
+    IF (step == 1) first_step_t = timer() - step_time
+    IF (step == 2) second_step_t = timer() - step_time
+    wall_clock = timer() - timerstart
+    step_clock = timer() - step_time
+    cells = REAL(x_cells, 8) * REAL(y_cells, 8)
+    WRITE(g_out, *) "Wall clock ", wall_clock
+    WRITE(0, *) "Wall clock ", wall_clock
+    WRITE(0, *) "Average time per cell ", wall_clock / (REAL(step, 8) * cells)
+    WRITE(g_out, *) "Average time per cell ", wall_clock / (REAL(step, 8) * cells)
+    WRITE(0, *) "Step time per cell    ", step_clock / cells
+    WRITE(g_out, *) "Step time per cell    ", step_clock / cells
+    diffuse_step = .FALSE.
+    IF (time + g_small > end_time .OR. step >= end_step) THEN
+      complete = .TRUE.
+      CALL field_summary()
+      wall_clock = timer() - timerstart
+      WRITE(g_out, *)
+      WRITE(g_out, *) 'Calculation complete'
+      WRITE(g_out, *) 'Tea is finishing'
+      WRITE(g_out, *) 'First step overhead', first_step_t - second_step_t
+      WRITE(g_out, *) 'Wall clock ', wall_clock
+      WRITE(0, *) 'First step overhead', first_step_t - second_step_t
+      WRITE(0, *) 'Wall clock ', wall_clock
+      IF (profiler_on) CALL profiler_report(wall_clock)
+      FLUSH(g_out)
+      diffuse_step = .TRUE.
+    ENDIF
+  END FUNCTION diffuse_step
+
+  SUBROUTINE profiler_report(wall_clock)
+    REAL(KIND=8), INTENT(IN) :: wall_clock
+    REAL(KIND=8) :: total
+    total = profiler%timestep + profiler%halo_exchange + profiler%summary + profiler%visit + &
+            profiler%tea_init + profiler%set_field + profiler%tea_solve + profiler%tea_reset + &
+            profiler%dot_product + profiler%halo_update + profiler%internal_halo_update
+    WRITE(g_out, *)
+    WRITE(g_out, '(a58,2f16.4)') "Profiler Output                 Time            Percentage"
+    WRITE(g_out, '(a23,2f16.4)') "Timestep              :", profiler%timestep, 100.0 * (profiler%timestep / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "MPI Halo Exchange     :", profiler%halo_exchange, 100.0 * (profiler%halo_exchange / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Self Halo Update      :", profiler%halo_update, 100.0 * (profiler%halo_update / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Internal Halo Update  :", profiler%internal_halo_update, &
+      100.0 * (profiler%internal_halo_update / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Summary               :", profiler%summary, 100.0 * (profiler%summary / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Visit                 :", profiler%visit, 100.0 * (profiler%visit / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Tea Init              :", profiler%tea_init, 100.0 * (profiler%tea_init / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Dot Product           :", profiler%dot_product, 100.0 * (profiler%dot_product / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Tea Solve             :", profiler%tea_solve, 100.0 * (profiler%tea_solve / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Tea Reset             :", profiler%tea_reset, 100.0 * (profiler%tea_reset / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Set Field             :", profiler%set_field, 100.0 * (profiler%set_field / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "Total                 :", total, 100.0 * (total / wall_clock)
+    WRITE(g_out, '(a23,2f16.4)') "The Rest              :", wall_clock - total, 100.0 * (wall_clock - total) / wall_clock
+  END SUBROUTINE profiler_report
+
+  SUBROUTINE shutdown()
+    IF (host_ready) THEN
+      CALL hip_check(tl_finalize(), 'finalize')
+      host_ready = .FALSE.
+    ENDIF
+    IF (out_is_file) THEN
+      CLOSE(g_out)
+      out_is_file = .FALSE.
+    ENDIF
+    g_out = 6
+    IF (ALLOCATED(states)) DEALLOCATE(states)
+  END SUBROUTINE shutdown
+
+  ! ===========================================================================
+  ! C entry points (used by the tea_leaf program, Python and benchmarks)
+  ! ===========================================================================
+  FUNCTION c_to_f(s, n) RESULT(f)
+    CHARACTER(KIND=c_char), INTENT(IN) :: s(*)
+    INTEGER(c_int), INTENT(IN) :: n
+    CHARACTER(LEN=n) :: f
+    INTEGER :: i
+    DO i = 1, n
+      f(i:i) = s(i)
+    ENDDO
+  END FUNCTION c_to_f
+
+  !> Read the deck, set up the device chunk and print the step-0 summary.
+  INTEGER(c_int) FUNCTION tea_host_init(deck, ndeck, outp, nout, device) BIND(C, name="tea_host_init")
+    CHARACTER(KIND=c_char), INTENT(IN) :: deck(*), outp(*)
+    INTEGER(c_int), VALUE :: ndeck, nout, device
+    IF (host_ready) CALL shutdown()
+    CALL initialise(c_to_f(deck, ndeck), c_to_f(outp, nout), INT(device))
+    tea_host_init = 0
+  END FUNCTION tea_host_init
+
+  !> One timestep; returns 1 when the run is complete, else 0.
+  INTEGER(c_int) FUNCTION tea_host_step() BIND(C, name="tea_host_step")
+    tea_host_step = -1
+    IF (.NOT. host_ready) RETURN
+    tea_host_step = MERGE(1_c_int, 0_c_int, diffuse_step())
+  END FUNCTION tea_host_step
+
+  !> Run to completion; returns the number of steps taken.
+  INTEGER(c_int) FUNCTION tea_host_run() BIND(C, name="tea_host_run")
+    tea_host_run = -1
+    IF (.NOT. host_ready) RETURN
+    DO
+      IF (diffuse_step()) EXIT
+    ENDDO
+    tea_host_run = INT(step, c_int)
+  END FUNCTION tea_host_run
+
+  !> Results of the last step: out(1:13) = step, time, iterations,
+  !> ppcg inner iterations, conduction error, exact error, solve seconds,
+  !> switch iteration, summary step, vol, mass, ie, temp.
+  INTEGER(c_int) FUNCTION tea_host_last(out) BIND(C, name="tea_host_last")
+    REAL(c_double), INTENT(OUT) :: out(13)
+    out(1) = step
+    out(2) = time
+    out(3) = last_iterations
+    out(4) = last_ppcg_inner
+    out(5) = last_cond_error
+    out(6) = last_exact_error
+    out(7) = last_solve_time
+    out(8) = last_switch
+    out(9) = last_summary_step
+    out(10) = last_summary(1)
+    out(11) = last_summary(2)
+    out(12) = last_summary(4)
+    out(13) = last_summary(5)
+    tea_host_last = 0
+  END FUNCTION tea_host_last
+
+  INTEGER(c_int) FUNCTION tea_host_finalize() BIND(C, name="tea_host_finalize")
+    CALL shutdown()
+    tea_host_finalize = 0
+  END FUNCTION tea_host_finalize
+
+END MODULE tea_host

@@ -1,0 +1,110 @@
+"""Capture the parity fixtures from the reference itself.
+
+Runs oracle/_ref/tea_leaf (the unmodified reference built from its sources by
+oracle/ref.mk) on one thread for every seeded deck below, and stores the
+parsed tea.out (per-step `step:` rows, iteration counts, switch/eigen lines)
+as JSON under tests/golden/fixtures/.  Also converts the goldens shipped with
+the reference (Benchmarks/tea_bm_N.{in,out}) into the same JSON form.
+
+    python tests/golden/make_fixtures.py [name ...]
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from tealeaf_ref_amd import teaout  # noqa: E402
+from tealeaf_ref_amd.deckgen import make_deck  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "tea_leaf")
+REF_SRC = "/root/reference"
+OUT = os.path.join(ROOT, "tests", "golden", "fixtures")
+
+CASES = {
+    "cfg1_cg_512": dict(cells=512, solver="cg", seed=1, eps=1e-15, max_iters=5000, end_step=10),
+    "cg_circles_128": dict(cells=128, solver="cg", seed=2, n_circ=2, end_step=5),
+    "cheby_100": dict(cells=100, solver="chebyshev", seed=3, n_circ=1, eps=1e-10, max_iters=2000,
+                      end_step=4),
+    "cheby_presteps_96": dict(cells=96, solver="chebyshev", seed=4, presteps=20, eps=1e-12,
+                              max_iters=1500, end_step=3),
+    "ppcg_128": dict(cells=128, solver="ppcg", seed=5, inner_steps=10, end_step=5),
+    "ppcg_default_inner_96": dict(cells=96, solver="ppcg", seed=6, end_step=4),
+    "ppcg_halo2_96": dict(cells=96, solver="ppcg", seed=6, inner_steps=8, halo_depth=2, end_step=4),
+    "jacobi_48": dict(cells=48, solver="jacobi", seed=7, eps=1e-6, max_iters=8000, end_step=3),
+    "cg_jacdiag_130x122": dict(cells=130, y_cells=122, solver="cg", seed=8, precond="jac_diag",
+                               end_step=4),
+    "cg_jacblock_130x122": dict(cells=130, y_cells=122, solver="cg", seed=8, precond="jac_block",
+                                end_step=4),
+    "cg_reflective_96": dict(cells=96, solver="cg", seed=9, reflective=True, end_step=4),
+    "cheby_reflective_96": dict(cells=96, solver="chebyshev", seed=9, reflective=True, eps=1e-10,
+                                max_iters=1500, end_step=3),
+    "ppcg_reflective_96": dict(cells=96, solver="ppcg", seed=9, reflective=True, inner_steps=6,
+                               end_step=3),
+    "jacobi_reflective_40": dict(cells=40, solver="jacobi", seed=9, reflective=True, eps=1e-6,
+                                 max_iters=8000, end_step=2),
+    "cg_invdensity_128": dict(cells=128, solver="cg", seed=10, coefficient="inverse_density",
+                              end_step=4),
+    "ppcg_jacdiag_96": dict(cells=96, solver="ppcg", seed=11, precond="jac_diag", inner_steps=8,
+                            end_step=3),
+    "cheby_jacdiag_96": dict(cells=96, solver="chebyshev", seed=11, precond="jac_diag", eps=1e-10,
+                             max_iters=1500, end_step=3),
+    "ppcg_jacblock_96": dict(cells=96, solver="ppcg", seed=13, precond="jac_block", inner_steps=6,
+                             end_step=3),
+    "cg_checkresult_64": dict(cells=64, solver="cg", seed=12, check_result=True, end_step=2),
+}
+
+
+def run_reference(deck):
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "tea.in"), "w") as f:
+            f.write(deck)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_BIN], cwd=d, env=env, check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, timeout=3600)
+        with open(os.path.join(d, "tea.out")) as f:
+            return f.read()
+
+
+def fixture_from(text, deck, source):
+    parsed = teaout.parse(text)
+    return {"source": source, "deck": deck, "summaries": parsed["summaries"],
+            "steps": {str(k): v for k, v in parsed["steps"].items()}}
+
+
+def main(names):
+    os.makedirs(OUT, exist_ok=True)
+    for name, kw in CASES.items():
+        if names and name not in names:
+            continue
+        kw = dict(kw)
+        cells = kw.pop("cells")
+        deck = make_deck(cells, **kw)
+        print("reference run:", name, flush=True)
+        fx = fixture_from(run_reference(deck), deck,
+                          "oracle/_ref/tea_leaf (reference built from source), OMP_NUM_THREADS=1")
+        with open(os.path.join(OUT, name + ".json"), "w") as f:
+            json.dump(fx, f, indent=1)
+    # goldens shipped with the reference (data, converted)
+    for i in range(1, 7):
+        name = f"tea_bm_{i}"
+        if names and name not in names:
+            continue
+        src_in = os.path.join(REF_SRC, "Benchmarks", name + ".in")
+        src_out = os.path.join(REF_SRC, "Benchmarks", name + ".out")
+        if not os.path.exists(src_out):
+            continue
+        with open(src_in) as f:
+            deck = f.read()
+        with open(src_out) as f:
+            text = f.read()
+        fx = fixture_from(text, deck, f"Benchmarks/{name}.out shipped with the reference")
+        with open(os.path.join(OUT, name + ".json"), "w") as f:
+            json.dump(fx, f, indent=1)
+        print("converted", name)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,37 @@
+"""The C ABI library loads on a machine without a GPU and exports every entry
+point include/tealeaf_hip.h declares; the host library exports its own."""
+import ctypes
+import os
+import re
+
+from tealeaf_ref_amd import _lib, host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "tealeaf_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(tl_\w+)\s*\(", text, re.M)))
+
+
+def test_every_declared_symbol_exported():
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_calls_fail_cleanly_without_init():
+    L = _lib.lib()
+    out = ctypes.c_double()
+    assert L.tl_calc_2norm(1, ctypes.byref(out)) == 2  # TL_ERR_STATE
+    assert b"tl_init" in L.tl_last_error()
+
+
+def test_host_library_exports():
+    L = host.host_lib()
+    for n in ("tea_host_init", "tea_host_step", "tea_host_run", "tea_host_last",
+              "tea_host_finalize"):
+        assert hasattr(L, n)
+    assert os.access(host.TEA_LEAF_BIN, os.X_OK)

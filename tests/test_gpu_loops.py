@@ -1,0 +1,208 @@
+"""Fused device-side solver loops (tl_cg_loop, tl_ppcg_loop, tl_jacobi_loop,
+tl_cheby_run) against the oracle's routines driven one iteration at a time as
+tea_solve.f90:152-414 does.  Iteration counts must be identical; fields and
+scalars agree to 1e-10 relative."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import tl_oracle as O
+from helpers import assert_close, assert_scalar, interior
+from tealeaf_ref_amd import _lib as L
+from test_gpu_kernels import Pair, pair_factory  # noqa: F401  (fixture)
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(p):
+    p.init()
+    p.o.calc_residual()
+    L.call("tl_calc_residual")
+    r0 = p.o.calc_2norm(1)
+    rro = p.o.cg_init(p.precond)
+    p.d.scalar("tl_cg_init")
+    return r0, rro
+
+
+def oracle_cg_loop(o, precond, n_first, n_last, rro, old_error, eps_abs, check_switch=0,
+                   epslim=1.0, presteps=25):
+    al, be = {}, {}
+    err = 0.0
+    for n in range(n_first, n_last + 1):
+        if n > n_first and check_switch and abs(old_error) <= epslim and n >= presteps:
+            return dict(n_end=n, reason=L.LOOP_SWITCH, rro=rro, error=err, al=al, be=be)
+        pw = o.cg_calc_w()
+        a = rro / pw
+        al[n] = a
+        rrn = o.cg_calc_ur(a, precond)
+        b = rrn / rro
+        be[n] = b
+        o.cg_calc_p(b, precond, 0)
+        rro = rrn
+        err = math.sqrt(abs(rrn))
+        if abs(err) < eps_abs:
+            return dict(n_end=n, reason=L.LOOP_CONVERGED, rro=rro, error=err, al=al, be=be)
+        old_error = err
+    return dict(n_end=n_last, reason=L.LOOP_MAXITERS, rro=rro, error=err, al=al, be=be)
+
+
+def run_gpu_cg_loop(n_first, n_last, rro, old_error, eps_abs, check_switch=0, epslim=1.0,
+                    presteps=25):
+    st = L.LoopState(n_first=n_first, n_last=n_last, eps_abs=eps_abs, check_switch=check_switch,
+                     epslim=epslim, presteps=presteps, rro=rro, old_error=old_error)
+    al = np.zeros(n_last + 2)
+    be = np.zeros(n_last + 2)
+    L.call("tl_cg_loop", C.byref(st), al.ctypes.data_as(C.POINTER(C.c_double)),
+           be.ctypes.data_as(C.POINTER(C.c_double)))
+    return st, al, be
+
+
+@pytest.mark.parametrize("precond", [1, 2, 3])
+@pytest.mark.parametrize("batch_case", ["converge", "maxiters"])
+def test_cg_loop(pair_factory, precond, batch_case):
+    p = pair_factory(96, 80, 1, seed=17, precond=precond)
+    r0, rro = _setup(p)
+    r0s = math.sqrt(r0)
+    eps_abs = 1e-9 * r0s if batch_case == "converge" else 1e-300
+    n_last = 400 if batch_case == "converge" else 37
+    ref = oracle_cg_loop(p.o, precond, 1, n_last, rro, r0, eps_abs)
+    st, al, be = run_gpu_cg_loop(1, n_last, rro, r0, eps_abs)
+    assert st.reason == ref["reason"]
+    assert st.n_end == ref["n_end"], (st.n_end, ref["n_end"])
+    assert_scalar(st.rro, ref["rro"], 1e-8, "rro")
+    for n in range(1, ref["n_end"] + 1):
+        assert_scalar(al[n], ref["al"][n], 1e-9, f"alpha {n}")
+        assert_scalar(be[n], ref["be"][n], 1e-8, f"beta {n}")
+    p.cmp("u", tol=1e-10)
+
+
+def test_cg_loop_switch_and_resume(pair_factory):
+    """Switch test (tea_solve.f90:167) stops at the top of the iteration, and a
+    second call continues with the state the first left."""
+    p = pair_factory(64, 64, 1, seed=23)
+    r0, rro = _setup(p)
+    eps_abs = 1e-12 * math.sqrt(r0)
+    # epslim large: the switch fires as soon as n >= presteps
+    ref = oracle_cg_loop(p.o, 1, 1, 300, rro, r0, eps_abs, check_switch=1, epslim=1e30,
+                         presteps=7)
+    st, al, be = run_gpu_cg_loop(1, 300, rro, r0, eps_abs, check_switch=1, epslim=1e30,
+                                 presteps=7)
+    assert ref["reason"] == L.LOOP_SWITCH and ref["n_end"] == 7
+    assert st.reason == L.LOOP_SWITCH and st.n_end == 7
+    for n in range(1, 7):
+        assert_scalar(al[n], ref["al"][n], 1e-9, f"alpha {n}")
+    p.cmp("u", tol=1e-10)
+    # resume plain CG from iteration 7 on both sides
+    ref2 = oracle_cg_loop(p.o, 1, 7, 300, ref["rro"], ref["error"], eps_abs)
+    st2, _, _ = run_gpu_cg_loop(7, 300, st.rro, st.old_error, eps_abs)
+    assert st2.n_end == ref2["n_end"] and st2.reason == ref2["reason"]
+    p.cmp("u", tol=1e-10)
+
+
+@pytest.mark.parametrize("precond", [1, 2])
+@pytest.mark.parametrize("reflective", [False, True])
+def test_ppcg_loop(pair_factory, precond, reflective):
+    p = pair_factory(77, 58, 1, seed=31, precond=precond, reflective=reflective)
+    r0, rro = _setup(p)
+    eigmin, eigmax, inner = 0.9, 40.0, 6
+    ch_a, ch_b, theta = O.calc_ch_coefs(eigmin, eigmax, inner)
+    # the switch block (tea_solve.f90:229-250) on both sides
+    p.o.ppcg_init(2, precond, 1)
+    L.call("tl_set_ppcg_active", 1)
+    p.d.scalar("tl_ppcg_init", 2)
+    sdr = [0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 0]
+    sd_only = [0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0]
+    p.o.ppcg_init_sd(theta, precond)
+    p.o.update_halo(sdr, 1, reflective)
+    for i in range(inner):
+        p.o.ppcg_inner(ch_a[i], ch_b[i], precond)
+        p.o.update_halo(sd_only, 1, reflective)
+    p.o.ppcg_update_z()
+    rro_o = p.o.ppcg_init(3, precond, 1)
+    p_only = [0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0]
+    p.o.update_halo(p_only, 1, reflective)  # tea_solve.f90:250
+    a = np.ascontiguousarray(ch_a)
+    b = np.ascontiguousarray(ch_b)
+    pa, pb = a.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double))
+    L.call("tl_ppcg_init_sd", theta)
+    L.call("tl_ppcg_inner", pa, pb, 1, inner)
+    L.call("tl_ppcg_update_z")
+    rro_d = p.d.scalar("tl_ppcg_init", 3)
+    assert_scalar(rro_d, rro_o, 1e-10, "rro step 3")
+    eps_abs = 1e-10 * math.sqrt(r0)
+    # oracle outer loop
+    n_end, err = None, 0.0
+    rro = rro_o
+    for n in range(2, 200):
+        pw = p.o.cg_calc_w()
+        p.o.ppcg_store_r()
+        rro = p.o.ppcg_calc_zrnorm(precond, 1)
+        p.o.cg_calc_ur(rro / pw, precond)
+        p.o.ppcg_init_sd(theta, precond)
+        p.o.update_halo(sdr, 1, reflective)
+        for i in range(inner):
+            p.o.ppcg_inner(ch_a[i], ch_b[i], precond)
+            p.o.update_halo(sd_only, 1, reflective)
+        p.o.ppcg_update_z()
+        rrn = p.o.ppcg_calc_rrn()
+        p.o.cg_calc_p(rrn / rro, precond, 1)
+        p.o.update_halo(p_only, 1, reflective)  # tea_solve.f90:384 with fields = P
+        rro = rrn
+        err = math.sqrt(abs(rrn))
+        if err < eps_abs:
+            n_end = n
+            break
+    assert n_end is not None
+    st = L.LoopState(n_first=2, n_last=199, eps_abs=eps_abs, rro=rro_d, old_error=0.0)
+    inner_iters = C.c_int(0)
+    L.call("tl_ppcg_loop", C.byref(st), pa, pb, inner, theta, C.byref(inner_iters))
+    assert st.reason == L.LOOP_CONVERGED
+    assert st.n_end == n_end, (st.n_end, n_end)
+    assert inner_iters.value == inner * (n_end - 1)
+    assert_scalar(st.error, err, 1e-6, "error")
+    p.cmp("u", tol=1e-10)
+    p.cmp("z", tol=1e-8)
+
+
+@pytest.mark.parametrize("reflective", [False, True])
+def test_jacobi_loop(pair_factory, reflective):
+    p = pair_factory(45, 50, 1, seed=37, reflective=reflective)
+    p.init()
+    errs = []
+    eps_abs = 1e-3
+    n_end = None
+    for n in range(1, 500):
+        e = math.sqrt(abs(p.o.jacobi_solve()))
+        errs.append(e)
+        if e < eps_abs:
+            n_end = n
+            break
+    assert n_end is not None
+    st = L.LoopState(n_first=1, n_last=499, eps_abs=eps_abs)
+    L.call("tl_jacobi_loop", C.byref(st))
+    assert st.reason == L.LOOP_CONVERGED and st.n_end == n_end
+    # sum |u - un| cancels: each term is ~1e-7 of u, so rounding of u at 1e-16
+    # moves it at ~1e-9 relative and the sum at up to ~1e-6
+    assert_scalar(st.error, errs[-1], 1e-5, "error")
+    p.cmp("u", tol=1e-10)
+
+
+def test_cheby_run(pair_factory):
+    p = pair_factory(64, 40, 1, seed=41)
+    _setup(p)
+    theta = 20.0
+    ch_a, ch_b, _ = O.calc_ch_coefs(1.0, 39.0, 30)
+    p.o.cheby_init(theta, 1)
+    L.call("tl_cheby_init", theta)
+    for s in range(1, 21):
+        p.o.cheby_iterate(ch_a[s - 1], ch_b[s - 1], 1)
+    rr_o = p.o.calc_2norm(1)
+    a = np.ascontiguousarray(ch_a)
+    b = np.ascontiguousarray(ch_b)
+    rr = C.c_double()
+    L.call("tl_cheby_run", a.ctypes.data_as(C.POINTER(C.c_double)),
+           b.ctypes.data_as(C.POINTER(C.c_double)), 1, 20, 1, C.byref(rr))
+    assert_scalar(rr.value, rr_o, 1e-8, "cheby residual")
+    p.cmp("u", tol=1e-10)
