@@ -52,6 +52,7 @@ struct State {
   int precond = TL_PREC_NONE;
   int reflective = 0;
   int ppcg_active = 0;
+  int zb_all = 1;          // last tl_common_init had all four faces external
   double rx = 0.0, ry = 0.0;
 
   // fields (two buffers for the ones that are ping-ponged)
@@ -107,7 +108,10 @@ struct Grid {
   int blocks = 0;          // launched blocks (<= MAX_GRID, multiple of NUM_XCD)
   long tiles = 0;
 };
-Grid grid_for(int nx, int ny);
+// Tiles of cols_per_tile columns and as many rows (<= max_rows) as give
+// about target_tiles tiles; at most MAX_GRID blocks walk them.
+Grid grid_for(int nx, int ny, int cols_per_tile = BLOCK, int target_tiles = MAX_GRID,
+              int max_rows = 32);
 
 // profiling helpers (no-ops unless enabled)
 void prof_begin(int cls, hipEvent_t *a);
@@ -158,7 +162,9 @@ struct TileIter {
   long t, t_end;
   __device__ TileIter(long tiles, int blocks) {
     const int b = blockIdx.x;
-    const int lb = (blocks % NUM_XCD == 0) ? (b % NUM_XCD) * (blocks / NUM_XCD) + b / NUM_XCD : b;
+    const bool remap = blocks > 0 && blocks % NUM_XCD == 0;
+    if (blocks < 0) blocks = -blocks;  // negative count: plain round-robin order
+    const int lb = remap ? (b % NUM_XCD) * (blocks / NUM_XCD) + b / NUM_XCD : b;
     const long base = tiles / blocks, rem = tiles % blocks;
     t = lb * base + (lb < rem ? lb : rem);
     t_end = t + base + (lb < rem ? 1 : 0);

@@ -3,15 +3,30 @@
 // tea_leaf_cheby.f90, tea_leaf_ppcg.f90, tea_leaf_jacobi.f90, update_halo.f90,
 // field_summary.f90).  Scalars come back synchronously, as the reference
 // wrappers return them.
+#include <cstdlib>
+
 #include "tl_internal.hpp"
 #include "tl_stencil.hpp"
 
 namespace tl {
 Field *field_by_id(int id);
+dev::Geo geo(int cols_per_tile = BLOCK);
 
-dev::Geo geo() {
+dev::Geo geo(int cols_per_tile) {
   State &s = S();
-  const Grid g = grid_for(s.nx, s.ny);
+  // the paired (512-column) kernels run best with ~1024 tall tiles: fewer
+  // concurrent row streams keep HBM pages open (measured on MI355X)
+  static int target = -1, maxrows = -1, remap = -1;
+  if (target < 0) {
+    const char *e = getenv("TL_TILES");
+    target = e ? atoi(e) : 1024;
+    e = getenv("TL_MAXROWS");
+    maxrows = e ? atoi(e) : 64;
+    e = getenv("TL_XCD_REMAP");
+    remap = e ? atoi(e) : 1;
+  }
+  const Grid g = cols_per_tile > BLOCK ? grid_for(s.nx, s.ny, cols_per_tile, target, maxrows)
+                                       : grid_for(s.nx, s.ny, cols_per_tile);
   dev::Geo G;
   G.nx = s.nx;
   G.ny = s.ny;
@@ -20,6 +35,8 @@ dev::Geo geo() {
   G.rows_per_tile = g.rows_per_tile;
   G.tiles = g.tiles;
   G.blocks = g.blocks;
+  // a negative count tells the kernels to walk tiles in plain block order
+  if (cols_per_tile > BLOCK && !remap) G.blocks = -g.blocks;
   return G;
 }
 
@@ -635,6 +652,7 @@ extern "C" int tl_common_init(double rx, double ry, const int zero_boundary[4]) 
   if (!zero_boundary) return fail(TL_ERR_ARG, "null zero_boundary");
   s.rx = rx;
   s.ry = ry;
+  s.zb_all = zero_boundary[0] && zero_boundary[1] && zero_boundary[2] && zero_boundary[3];
   G_LAUNCH(k_init_u, s.energy1.p11, s.density.p11, s.u[s.ucur].p11, s.u0.p11);
   const long nk = (long)(s.nx + 2 * s.h - 1) * (s.ny + 2 * s.h - 1);
   hipLaunchKernelGGL(k_init_k, dim3(flat_blocks(nk)), dim3(BLOCK), 0, s.stream, s.nx, s.ny, s.h,

@@ -13,9 +13,10 @@
 
 #include "tl_internal.hpp"
 #include "tl_stencil.hpp"
+#include "tl_stencil2.hpp"
 
 namespace tl {
-dev::Geo geo();
+dev::Geo geo(int cols_per_tile);
 }
 using namespace tl;
 using dev::Geo;
@@ -50,6 +51,7 @@ struct LoopParams {
 // w = A p, pw partials] then K2(n) [alpha = rro/pw, u += alpha p,
 // r -= alpha w, (z = Mi r), rrn partials].
 // ---------------------------------------------------------------------------
+template <bool NT>
 __global__ __launch_bounds__(BLOCK) void k_cgl_w(Geo G, int n, int first, int tail,
                                                   LoopParams lp, double rro_in,
                                                   const double *__restrict__ src,
@@ -62,6 +64,7 @@ __global__ __launch_bounds__(BLOCK) void k_cgl_w(Geo G, int n, int first, int ta
                                                   double *__restrict__ out_part, Ctl *ctl,
                                                   double *__restrict__ hrro,
                                                   double *__restrict__ hbeta) {
+  using dev::d2;
   __shared__ double lds[BLOCK / WAVE];
   double beta = 0.0;
   bool stop = false;
@@ -95,36 +98,44 @@ __global__ __launch_bounds__(BLOCK) void k_cgl_w(Geo G, int n, int first, int ta
   // p = src + beta p of iteration n-1 (tea_solve.f90:366) is applied even when
   // the loop stops here, so p ends exactly as the reference leaves it.
   const bool upd = !first;
+  const long P = G.pitch;
   double acc = 0.0;
-  TILE_LOOP(G)
-  if (stop) {
-    for (int y = tt.y0; y < tt.y1; ++y) {
-      const long i = IDX(y, x);
-      pout[i] = src[i] + beta * pin[i];
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::Tile2 tt = dev::tile2_of(G, t);
+    if (stop) {
+      if (tt.c < G.nx)
+        for (int y = tt.y0; y < tt.y1; ++y) {
+          const long i = (long)y * P + tt.c;
+          const d2 v = dev::ld2<NT>(src + i) + beta * dev::ld2<NT>(pin + i);
+          dev::st_pair<NT>(pout + (long)y * P, tt.c, G.nx, v);
+        }
+      continue;
     }
-  } else {
-    dev::walk<MIRROR>(
-        G, x, tt.y0, tt.y1,
-        [&](int xx, int yy) {
-          const long i = IDX(yy, xx);
-          return upd ? src[i] + beta * pin[i] : pin[i];
-        },
-        Kx, Ky,
-        [&](int y, double c, double l, double rr, double d, double up, double kxl, double kxr,
-            double kyd, double kyu) {
-          const long i = IDX(y, x);
-          const double wv = dev::apply_a(c, l, rr, d, up, kxl, kxr, kyd, kyu, rx, ry);
-          w[i] = wv;
-          pout[i] = c;
-          acc += wv * c;
-        });
+    auto ldp = [&](int col, int row) -> d2 {
+      const long i = (long)row * P + col;
+      const d2 pv = dev::ld2<NT>(pin + i);
+      return upd ? dev::ld2<NT>(src + i) + beta * pv : pv;
+    };
+    auto ld1p = [&](int col, int row) -> double {
+      const long i = (long)row * P + col;
+      return upd ? src[i] + beta * pin[i] : pin[i];
+    };
+    dev::walk2<NT>(G, tt, Kx, Ky, rx, ry, ldp, ld1p, [&](int y, d2 v, d2 wv) {
+      if (tt.c < G.nx) {
+        dev::st_pair<NT>(w + (long)y * P, tt.c, G.nx, wv);
+        dev::st_pair<NT>(pout + (long)y * P, tt.c, G.nx, v);
+        acc += wv.x * v.x;
+        if (tt.c + 1 < G.nx) acc += wv.y * v.y;
+      }
+    });
   }
-  TILE_LOOP_END
   if (stop) return;
   acc = dev::block_sum(acc, lds);
   if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
 }
 
+template <bool NT>
 __global__ __launch_bounds__(BLOCK) void k_cgl_ur(Geo G, int n, double *__restrict__ u,
                                                    const double *__restrict__ p,
                                                    double *__restrict__ r,
@@ -135,27 +146,36 @@ __global__ __launch_bounds__(BLOCK) void k_cgl_ur(Geo G, int n, double *__restri
                                                    double *__restrict__ out_part, Ctl *ctl,
                                                    const double *__restrict__ hrro,
                                                    double *__restrict__ halpha) {
+  using dev::d2;
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
   const double pw = dev::sum_partials(prev_part, nprev, lds);
   const double alpha = hrro[n] / pw;                    // tea_solve.f90:352
   if (blockIdx.x == 0 && threadIdx.x == 0) halpha[n] = alpha;
+  const long P = G.pitch;
   double acc = 0.0;
-  TILE_LOOP(G)
-  for (int y = tt.y0; y < tt.y1; ++y) {
-    const long i = IDX(y, x);
-    u[i] = u[i] + alpha * p[i];
-    const double rv = r[i] - alpha * w[i];
-    r[i] = rv;
-    if (diag) {
-      const double zv = Mi[i] * rv;
-      z[i] = zv;
-      acc += rv * zv;
-    } else {
-      acc += rv * rv;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::Tile2 tt = dev::tile2_of(G, t);
+    if (tt.c >= G.nx) continue;
+    const bool pair = tt.c + 1 < G.nx;
+    for (int y = tt.y0; y < tt.y1; ++y) {
+      const long i = (long)y * P + tt.c;
+      const d2 uv = dev::ld2<NT>(u + i) + alpha * dev::ld2<NT>(p + i);
+      const d2 rv = dev::ld2<NT>(r + i) - alpha * dev::ld2<NT>(w + i);
+      dev::st_pair<NT>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::st_pair<NT>(r + (long)y * P, tt.c, G.nx, rv);
+      if (diag) {
+        const d2 zv = dev::ld2<NT>(Mi + i) * rv;
+        dev::st_pair<NT>(z + (long)y * P, tt.c, G.nx, zv);
+        acc += rv.x * zv.x;
+        if (pair) acc += rv.y * zv.y;
+      } else {
+        acc += rv.x * rv.x;
+        if (pair) acc += rv.y * rv.y;
+      }
     }
   }
-  TILE_LOOP_END
   acc = dev::block_sum(acc, lds);
   if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
 }
@@ -373,6 +393,19 @@ __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
+// Nontemporal (streaming) accesses pay off once the working set of a
+// loop (~6 fields) no longer fits the 256 MB Infinity Cache.
+bool use_nontemporal() {
+  static int forced = -2;
+  if (forced == -2) {
+    const char *e = getenv("TL_NT");
+    forced = e ? atoi(e) : -1;
+  }
+  if (forced >= 0) return forced != 0;
+  const State &s = S();
+  return (double)s.nx * (double)s.ny * 8.0 * 6.0 > 256.0 * 1024 * 1024;
+}
+
 int batch_size() {
   static int b = 0;
   if (!b) {
@@ -488,8 +521,9 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
   int rc = check_state(st);
   if (rc) return rc;
   State &s = S();
-  if (s.precond == TL_PREC_JAC_BLOCK) {
-    // block-tridiagonal preconditioner: the per-routine path, one iteration at a time
+  if (s.precond == TL_PREC_JAC_BLOCK || !s.zb_all) {
+    // block-tridiagonal preconditioner, or a chunk with internal faces: the
+    // per-routine path, one iteration at a time
     double rro = st->rro, old_error = st->old_error, err = 0.0;
     for (int n = st->n_first; n <= st->n_last; ++n) {
       if (n > st->n_first && st->check_switch && fabs(old_error) <= st->epslim &&
@@ -519,10 +553,13 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
     return TL_OK;
   }
   if ((rc = ensure_loop_buffers(st->n_last))) return rc;
-  const Geo G = geo();
-  const int nb = G.blocks;
+  const Geo G = geo(dev::COLS_PER_BLOCK);
+  const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int diag = s.precond == TL_PREC_JAC_DIAG;
   const double *src = (diag || s.ppcg_active) ? s.z.p11 : s.r.p11;
+  const bool nt = use_nontemporal();
+  auto kw = nt ? k_cgl_w<true> : k_cgl_w<false>;
+  auto kur = nt ? k_cgl_ur<true> : k_cgl_ur<false>;
   const int c0 = s.pcur;
   LoopParams lp;
   lp.eps_abs = st->eps_abs;
@@ -534,12 +571,12 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
     const int cin = c0 ^ ((n - st->n_first) & 1);
     hipEvent_t ev;
     prof_begin(0, &ev);
-    hipLaunchKernelGGL(k_cgl_w, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
+    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
                        st->rro, src, s.p[cin].p11, s.p[1 - cin].p11, s.Kx.p11, s.Ky.p11, s.w.p11,
                        s.rx, s.ry, part(P2), nb, part(P1), ctl, h_rro(), h_beta());
     prof_end(0, ev);
     prof_begin(1, &ev);
-    hipLaunchKernelGGL(k_cgl_ur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, s.u[s.ucur].p11,
+    hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, s.u[s.ucur].p11,
                        s.p[1 - cin].p11, s.r.p11, s.w.p11, s.z.p11, s.Mi.p11, diag, part(P1), nb,
                        part(P2), ctl, h_rro(), h_alpha());
     prof_end(1, ev);
@@ -548,7 +585,7 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
   };
   auto tail = [&](int n) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
-    hipLaunchKernelGGL(k_cgl_w, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, st->rro, src,
+    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, st->rro, src,
                        s.p[cin].p11, s.p[1 - cin].p11, s.Kx.p11, s.Ky.p11, s.w.p11, s.rx, s.ry,
                        part(P2), nb, part(P1), ctl, h_rro(), h_beta());
     TL_HIP(hipGetLastError());
@@ -608,7 +645,7 @@ extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const doubl
     return TL_OK;
   }
   if ((rc = ensure_loop_buffers(st->n_last))) return rc;
-  const Geo G = geo();
+  const Geo G = geo(BLOCK);
   const int nb = G.blocks;
   const int diag = s.precond == TL_PREC_JAC_DIAG;
   const int c0 = s.pcur;
@@ -672,7 +709,7 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   if (rc) return rc;
   if ((rc = ensure_loop_buffers(st->n_last))) return rc;
   State &s = S();
-  const Geo G = geo();
+  const Geo G = geo(BLOCK);
   const int nb = G.blocks;
   const int u0c = s.ucur;
   Ctl *ctl = (Ctl *)s.ctl;

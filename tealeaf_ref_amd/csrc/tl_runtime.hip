@@ -23,12 +23,12 @@ int hip_fail(hipError_t e, const char *what) {
   return TL_ERR_HIP;
 }
 
-Grid grid_for(int nx, int ny) {
+Grid grid_for(int nx, int ny, int cols_per_tile, int target_tiles, int max_rows) {
   Grid g;
-  g.tiles_x = (nx + BLOCK - 1) / BLOCK;
-  long r = ((long)g.tiles_x * ny) / MAX_GRID;
+  g.tiles_x = (nx + cols_per_tile - 1) / cols_per_tile;
+  long r = ((long)g.tiles_x * ny) / target_tiles;
   if (r < 1) r = 1;
-  if (r > 32) r = 32;
+  if (r > max_rows) r = max_rows;
   g.rows_per_tile = (int)r;
   g.tiles_y = (ny + g.rows_per_tile - 1) / g.rows_per_tile;
   g.tiles = (long)g.tiles_x * g.tiles_y;
@@ -192,7 +192,8 @@ extern "C" int tl_init(int nx, int ny, int halo, int device) {
   s.ny = ny;
   s.h = halo;
   s.device = device;
-  s.pitch = ((long)PADL + nx + halo + 31) / 32 * 32;
+  // two spare columns past the halo: paired loads may touch column nx+1
+  s.pitch = ((long)PADL + nx + halo + 2 + 31) / 32 * 32;
   s.rows = (long)ny + 2L * halo;
   s.field_bytes = sizeof(double) * (size_t)(s.pitch * s.rows + 64);
   TL_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
