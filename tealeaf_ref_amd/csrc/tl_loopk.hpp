@@ -1,0 +1,539 @@
+// Fused solver loops shared by the FP64 entry points (tl_loops.hip) and the
+// FP32 inner solves of the mixed-precision solver (tl_mixed.hip).
+//
+// Every reduction is finished redundantly by every block of the *next*
+// kernel: block b of kernel k+1 sums the per-block partials kernel k wrote,
+// in one fixed order, so all blocks see the same bits.  That removes the host
+// round trip per dot product and needs no atomics or cross-XCD fences;
+// kernel boundaries order everything.  The loop control block (stop flag,
+// reason, iteration) is written by block 0 of the kernel that decides; the
+// kernels of an in-flight batch that follow see the flag and return.
+#pragma once
+
+#include <cmath>
+#include <cstdlib>
+
+#include "tl_internal.hpp"
+#include "tl_stencil.hpp"
+#include "tl_stencilv.hpp"
+
+namespace tl {
+
+struct Ctl {
+  int stop, reason, n_end, pad;
+  double error, old_error, rro;
+};
+
+struct LoopParams {
+  double eps_abs;
+  double epslim;
+  int check_switch;
+  int presteps;
+};
+
+bool use_nontemporal();
+int batch_size();
+int ensure_loop_buffers(long n_last);
+double *h_rro();
+double *h_alpha();
+double *h_beta();
+int copy_hist(double *dst, const double *src_dev, int lo, int hi);
+
+// Drive a loop: launch(n, first) queues iteration n; tail(n) queues the exit
+// test of iteration n-1.  Batches of iterations are queued two deep and the
+// control block is polled once per batch.
+template <typename LAUNCH, typename TAIL>
+int drive(int n_first, int n_last, LAUNCH launch, TAIL tail, Ctl *out) {
+  State &s = S();
+  Ctl *hc = (Ctl *)s.hctl;
+  const int B = batch_size();
+  hipEvent_t ev[2];
+  TL_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  TL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int n = n_first, k = 0, rc = TL_OK;
+  bool stopped = false;
+  while (n <= n_last) {
+    const int m = (n + B - 1 < n_last) ? n + B - 1 : n_last;
+    for (int it = n; it <= m; ++it) {
+      rc = launch(it, it == n_first);
+      if (rc) goto done;
+    }
+    if (hipMemcpyAsync(&hc[k & 1], s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s.stream) !=
+            hipSuccess ||
+        hipEventRecord(ev[k & 1], s.stream) != hipSuccess) {
+      rc = fail(TL_ERR_HIP, "queueing the loop control block copy");
+      goto done;
+    }
+    if (k >= 1) {
+      if (hipEventSynchronize(ev[(k - 1) & 1]) != hipSuccess) {
+        rc = fail(TL_ERR_HIP, "hipEventSynchronize");
+        goto done;
+      }
+      if (hc[(k - 1) & 1].stop) {
+        stopped = true;
+        break;
+      }
+    }
+    n = m + 1;
+    ++k;
+  }
+  if (!stopped) {
+    rc = tail(n_last + 1);
+    if (rc) goto done;
+  }
+  if (hipMemcpyAsync(&hc[2], s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipStreamSynchronize(s.stream) != hipSuccess) {
+    rc = fail(TL_ERR_HIP, "reading the loop control block");
+    goto done;
+  }
+  *out = hc[2];
+done:
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// kernels (one copy per translation unit)
+// ---------------------------------------------------------------------------
+namespace {
+
+using dev::Geo;
+
+// Scalar head of the first kernel of an iteration: beta of iteration n-1,
+// exit test (tea_solve.f90:402-410) and, for CG, the switch test (:167).
+struct Head {
+  double beta;
+  bool stop;
+};
+
+__device__ inline Head loop_head(int n, int first, int tail, const LoopParams &lp,
+                                 double rro_in, const double *__restrict__ prev_part, int nprev,
+                                 Ctl *ctl, double *__restrict__ hrro,
+                                 double *__restrict__ hbeta, double *lds) {
+  Head h;
+  h.beta = 0.0;
+  h.stop = false;
+  if (!first) {
+    const double rrn = dev::sum_partials(prev_part, nprev, lds);
+    const double rro = hrro[n - 1];
+    h.beta = rrn / rro;
+    const double err = sqrt(fabs(rrn));
+    const bool conv = fabs(err) < lp.eps_abs;
+    const bool sw = !conv && !tail && lp.check_switch && fabs(err) <= lp.epslim &&
+                    n >= lp.presteps;
+    h.stop = conv || sw || tail;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      hbeta[n - 1] = h.beta;
+      if (h.stop) {
+        ctl->stop = 1;
+        ctl->reason = conv ? TL_LOOP_CONVERGED : (sw ? TL_LOOP_SWITCH : TL_LOOP_MAXITERS);
+        ctl->n_end = sw ? n : n - 1;
+        ctl->error = err;
+        ctl->old_error = err;
+        ctl->rro = rrn;
+      } else {
+        hrro[n] = rrn;
+      }
+    }
+  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->stop = 0;
+    hrro[n] = rro_in;
+  }
+  return h;
+}
+
+// First kernel of a CG or PPCG iteration n: p = src + beta p (of n-1; src is
+// r for CG, z for PPCG and preconditioned CG), w = A p, partials of p.w.
+// On the launch that stops the loop only the p update is done, so p ends as
+// the reference leaves it.
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int tail,
+                                                   LoopParams lp, double rro_in,
+                                                   const T *__restrict__ src,
+                                                   const T *__restrict__ pin,
+                                                   T *__restrict__ pout,
+                                                   const T *__restrict__ Kx,
+                                                   const T *__restrict__ Ky, T *__restrict__ w,
+                                                   T rx, T ry,
+                                                   const double *__restrict__ prev_part,
+                                                   int nprev, double *__restrict__ out_part,
+                                                   Ctl *ctl, double *__restrict__ hrro,
+                                                   double *__restrict__ hbeta) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (!first && ctl->stop) return;
+  const Head h = loop_head(n, first, tail, lp, rro_in, prev_part, nprev, ctl, hrro, hbeta, lds);
+  const T beta = (T)h.beta;
+  const bool upd = !first;
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    if (h.stop) {
+      if (tt.c < G.nx)
+        for (int y = tt.y0; y < tt.y1; ++y) {
+          const long i = (long)y * P + tt.c;
+          const VT v = dev::ldv<NT, T, V>(src + i) + beta * dev::ldv<NT, T, V>(pin + i);
+          dev::stv<NT, T, V>(pout + (long)y * P, tt.c, G.nx, v);
+        }
+      continue;
+    }
+    auto ldop = [&](int col, int row) -> VT {
+      const long i = (long)row * P + col;
+      const VT pv = dev::ldv<NT, T, V>(pin + i);
+      return upd ? dev::ldv<NT, T, V>(src + i) + beta * pv : pv;
+    };
+    auto ld1 = [&](int col, int row) -> T {
+      const long i = (long)row * P + col;
+      return upd ? src[i] + beta * pin[i] : pin[i];
+    };
+    dev::walkv<NT, T, V>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
+      if (tt.c < G.nx) {
+        dev::stv<NT, T, V>(w + (long)y * P, tt.c, G.nx, wv);
+        dev::stv<NT, T, V>(pout + (long)y * P, tt.c, G.nx, v);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (tt.c + e < G.nx) acc += (double)wv[e] * (double)v[e];
+      }
+    });
+  }
+  if (h.stop) return;
+  acc = dev::block_sum(acc, lds);
+  if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+}
+
+// CG iteration n, second pass: alpha = rro/pw; u += alpha p; r -= alpha w;
+// (z = Mi r); partials of r.r or r.z (kernels/tea_leaf_cg_kernel.f90:237-300)
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u,
+                                                  const T *__restrict__ p, T *__restrict__ r,
+                                                  const T *__restrict__ w, T *__restrict__ z,
+                                                  const T *__restrict__ Mi, int diag,
+                                                  const double *__restrict__ prev_part, int nprev,
+                                                  double *__restrict__ out_part, Ctl *ctl,
+                                                  const double *__restrict__ hrro,
+                                                  double *__restrict__ halpha) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl->stop) return;
+  const double pw = dev::sum_partials(prev_part, nprev, lds);
+  const double alpha = hrro[n] / pw;
+  if (blockIdx.x == 0 && threadIdx.x == 0) halpha[n] = alpha;
+  const T a = (T)alpha;
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    if (tt.c >= G.nx) continue;
+    for (int y = tt.y0; y < tt.y1; ++y) {
+      const long i = (long)y * P + tt.c;
+      const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
+      const VT rv = dev::ldv<NT, T, V>(r + i) - a * dev::ldv<NT, T, V>(w + i);
+      dev::stv<NT, T, V>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::stv<NT, T, V>(r + (long)y * P, tt.c, G.nx, rv);
+      if (diag) {
+        const VT zv = dev::ldv<NT, T, V>(Mi + i) * rv;
+        dev::stv<NT, T, V>(z + (long)y * P, tt.c, G.nx, zv);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (tt.c + e < G.nx) acc += (double)rv[e] * (double)zv[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (tt.c + e < G.nx) acc += (double)rv[e] * (double)rv[e];
+      }
+    }
+  }
+  acc = dev::block_sum(acc, lds);
+  if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+}
+
+// PPCG outer iteration n, second pass (tea_solve.f90:300-309): rro = z.r
+// (partials from the previous iteration's last sweep, or rro_in on the
+// first), alpha = rro/pw; u += alpha p; r_store = r; r -= alpha w; (z = Mi r)
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double rro_in,
+                                                  T *__restrict__ u, const T *__restrict__ p,
+                                                  T *__restrict__ r, T *__restrict__ r_store,
+                                                  const T *__restrict__ w, T *__restrict__ z,
+                                                  const T *__restrict__ Mi, int diag,
+                                                  const double *__restrict__ part_pw,
+                                                  const double *__restrict__ part_zr, int nprev,
+                                                  Ctl *ctl, double *__restrict__ hrro,
+                                                  double *__restrict__ halpha) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl->stop) return;
+  const double pw = dev::sum_partials(part_pw, nprev, lds);
+  const double rro = first ? rro_in : dev::sum_partials(part_zr, nprev, lds);
+  const double alpha = rro / pw;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hrro[n] = rro;
+    halpha[n] = alpha;
+  }
+  const T a = (T)alpha;
+  const long P = G.pitch;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    if (tt.c >= G.nx) continue;
+    for (int y = tt.y0; y < tt.y1; ++y) {
+      const long i = (long)y * P + tt.c;
+      const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
+      const VT ro = dev::ldv<NT, T, V>(r + i);
+      const VT rv = ro - a * dev::ldv<NT, T, V>(w + i);
+      dev::stv<NT, T, V>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::stv<NT, T, V>(r_store + (long)y * P, tt.c, G.nx, ro);
+      dev::stv<NT, T, V>(r + (long)y * P, tt.c, G.nx, rv);
+      if (diag) dev::stv<NT, T, V>(z + (long)y * P, tt.c, G.nx, dev::ldv<NT, T, V>(Mi + i) * rv);
+    }
+  }
+}
+
+// One inner sweep of the polynomial preconditioner (kernels/tea_leaf_ppcg_kernel.f90:288-306):
+// rtemp -= A sd; sd' = alpha sd + beta (Mi) rtemp; utemp += sd'.  The first
+// sweep forms sd = src/theta, rtemp = r, utemp = sd on the fly
+// (tea_leaf_ppcg_init_sd); the last writes z = utemp
+// (tea_leaf_ppcg_update_z) and the partials of (r - r_store).z
+// (tea_leaf_ppcg_calc_rrn) and of z.r (the next iteration's zrnorm).
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_pp_sweep(
+    Geo G, int first_sweep, int last_sweep, int want_rrn, T theta_r, T alpha, T beta, int diag,
+    const T *__restrict__ src, const T *__restrict__ sdin, T *__restrict__ sdout,
+    T *__restrict__ rtemp, T *__restrict__ utemp, const T *__restrict__ r,
+    const T *__restrict__ r_store, T *__restrict__ z, const T *__restrict__ Mi,
+    const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry,
+    double *__restrict__ part_rrn, double *__restrict__ part_zr, Ctl *ctl) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl->stop) return;
+  const long P = G.pitch;
+  double arrn = 0.0, azr = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    auto ldop = [&](int col, int row) -> VT {
+      const long i = (long)row * P + col;
+      return first_sweep ? dev::ldv<NT, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
+    };
+    auto ld1 = [&](int col, int row) -> T {
+      const long i = (long)row * P + col;
+      return first_sweep ? src[i] * theta_r : sdin[i];
+    };
+    dev::walkv<NT, T, V>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
+      if (tt.c >= G.nx) return;
+      const long i = (long)y * P + tt.c;
+      const VT rv = (first_sweep || last_sweep) ? dev::ldv<NT, T, V>(r + i) : VT(0);
+      const VT rt0 = first_sweep ? rv : dev::ldv<NT, T, V>(rtemp + i);
+      const VT ut0 = first_sweep ? v : dev::ldv<NT, T, V>(utemp + i);
+      const VT rt = rt0 - wv;
+      const VT sv = diag ? alpha * v + (beta * dev::ldv<NT, T, V>(Mi + i)) * rt : alpha * v + beta * rt;
+      const VT ut = ut0 + sv;
+      dev::stv<NT, T, V>(rtemp + (long)y * P, tt.c, G.nx, rt);
+      dev::stv<NT, T, V>(sdout + (long)y * P, tt.c, G.nx, sv);
+      dev::stv<NT, T, V>(utemp + (long)y * P, tt.c, G.nx, ut);
+      if (last_sweep) {
+        dev::stv<NT, T, V>(z + (long)y * P, tt.c, G.nx, ut);
+        const VT rs = want_rrn ? dev::ldv<NT, T, V>(r_store + i) : VT(0);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (tt.c + e < G.nx) {
+            if (want_rrn) arrn += ((double)rv[e] - (double)rs[e]) * (double)ut[e];
+            azr += (double)ut[e] * (double)rv[e];
+          }
+      }
+    });
+  }
+  if (last_sweep) {
+    arrn = dev::block_sum(arrn, lds);
+    azr = dev::block_sum(azr, lds);
+    if (threadIdx.x == 0) {
+      part_rrn[blockIdx.x] = arrn;
+      part_zr[blockIdx.x] = azr;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host loops over one set of buffers (FP64 fields or the FP32 inner fields)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LoopBufs {
+  Geo G;                  // geometry; G.pitch in elements of T
+  T *u, *r, *w, *z, *r_store, *rtemp, *utemp;
+  T *p[2];
+  T *sd[2];
+  int *pcur, *sdcur;
+  const T *Mi, *Kx, *Ky;
+  T rx, ry;
+  int diag;               // Jacobi-diagonal preconditioner
+  const T *cg_src;        // r, or z when preconditioned: the CG direction source
+};
+
+inline int abs_blocks(const Geo &G) { return G.blocks < 0 ? -G.blocks : G.blocks; }
+
+// CG iterations (tea_solve.f90:341-369); see tl_cg_loop.  pin_first, when
+// given, replaces p[pcur] as the direction of the first iteration.
+template <typename T, int V>
+int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, double *cg_betas,
+                const T *pin_first = nullptr) {
+  State &s = S();
+  int rc;
+  if ((rc = ensure_loop_buffers(st->n_last))) return rc;
+  const Geo G = b.G;
+  const int nb = abs_blocks(G);
+  const bool nt = use_nontemporal();
+  auto kw = nt ? k_loop_w<true, T, V> : k_loop_w<false, T, V>;
+  auto kur = nt ? k_cg_ur<true, T, V> : k_cg_ur<false, T, V>;
+  const int c0 = *b.pcur;
+  LoopParams lp;
+  lp.eps_abs = st->eps_abs;
+  lp.epslim = st->epslim;
+  lp.check_switch = st->check_switch;
+  lp.presteps = st->presteps;
+  Ctl *ctl = (Ctl *)s.ctl;
+  auto launch = [&](int n, bool first) -> int {
+    const int cin = c0 ^ ((n - st->n_first) & 1);
+    const T *pin = (first && pin_first) ? pin_first : b.p[cin];
+    hipEvent_t ev;
+    prof_begin(0, &ev);
+    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
+                       st->rro, b.cg_src, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
+                       part(P2), nb, part(P1), ctl, h_rro(), h_beta());
+    prof_end(0, ev);
+    prof_begin(1, &ev);
+    hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
+                       b.w, b.z, b.Mi, b.diag, part(P1), nb, part(P2), ctl, h_rro(), h_alpha());
+    prof_end(1, ev);
+    TL_HIP(hipGetLastError());
+    return TL_OK;
+  };
+  auto tail = [&](int n) -> int {
+    const int cin = c0 ^ ((n - st->n_first) & 1);
+    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, st->rro, b.cg_src,
+                       b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P2), nb,
+                       part(P1), ctl, h_rro(), h_beta());
+    TL_HIP(hipGetLastError());
+    return TL_OK;
+  };
+  Ctl res;
+  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res))) return rc;
+  st->n_end = res.n_end;
+  st->reason = res.reason;
+  st->error = res.error;
+  st->rro = res.rro;
+  st->old_error = res.old_error;
+  // the launch that stopped the loop applied the last p update
+  const int n_det = res.reason == TL_LOOP_SWITCH ? res.n_end : res.n_end + 1;
+  *b.pcur = c0 ^ ((n_det - st->n_first + 1) & 1);
+  const int last_done = res.reason == TL_LOOP_SWITCH ? res.n_end - 1 : res.n_end;
+  if ((rc = copy_hist(cg_alphas, h_alpha(), st->n_first, last_done))) return rc;
+  if ((rc = copy_hist(cg_betas, h_beta(), st->n_first, last_done))) return rc;
+  return TL_OK;
+}
+
+// PPCG outer iterations after the switch (tea_solve.f90:289-331); see
+// tl_ppcg_loop.  rro_first is z.r entering n_first; pin_first, when given,
+// replaces p[pcur] as the direction of the first iteration.
+template <typename T, int V>
+int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
+                  const double *betas, int inner, double theta, double rro_first,
+                  const T *pin_first, int *inner_iters) {
+  State &s = S();
+  int rc;
+  if ((rc = ensure_loop_buffers(st->n_last))) return rc;
+  const Geo G = b.G;
+  const int nb = abs_blocks(G);
+  const bool nt = use_nontemporal();
+  auto kw = nt ? k_loop_w<true, T, V> : k_loop_w<false, T, V>;
+  auto kur = nt ? k_pp_ur<true, T, V> : k_pp_ur<false, T, V>;
+  auto ksw = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
+  const int c0 = *b.pcur;
+  int sd = *b.sdcur;
+  Ctl *ctl = (Ctl *)s.ctl;
+  LoopParams lp;
+  lp.eps_abs = st->eps_abs;
+  lp.epslim = 0.0;
+  lp.check_switch = 0;
+  lp.presteps = 0;
+  const T theta_r = (T)(1.0 / theta);
+  const T *src_sd = b.diag ? b.z : b.r;
+  auto launch = [&](int n, bool first) -> int {
+    const int cin = c0 ^ ((n - st->n_first) & 1);
+    const T *pin = (first && pin_first) ? pin_first : b.p[cin];
+    hipEvent_t ev;
+    prof_begin(5, &ev);
+    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
+                       rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3),
+                       nb, part(P1), ctl, h_rro(), h_beta());
+    prof_end(5, ev);
+    prof_begin(6, &ev);
+    hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, rro_first,
+                       b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag, part(P1),
+                       part(P2), nb, ctl, h_rro(), h_alpha());
+    prof_end(6, ev);
+    for (int i = 1; i <= inner; ++i) {
+      prof_begin(2, &ev);
+      hipLaunchKernelGGL(ksw, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
+                         i == inner ? 1 : 0, 1, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
+                         b.diag, src_sd, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r, b.r_store,
+                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2), ctl);
+      prof_end(2, ev);
+      sd = 1 - sd;
+    }
+    TL_HIP(hipGetLastError());
+    return TL_OK;
+  };
+  auto tail = [&](int n) -> int {
+    const int cin = c0 ^ ((n - st->n_first) & 1);
+    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, rro_first, b.z,
+                       b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3), nb,
+                       part(P1), ctl, h_rro(), h_beta());
+    TL_HIP(hipGetLastError());
+    return TL_OK;
+  };
+  Ctl res;
+  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res))) return rc;
+  st->n_end = res.n_end;
+  st->reason = res.reason;
+  st->error = res.error;
+  st->rro = res.rro;
+  st->old_error = res.old_error;
+  const int executed = res.n_end - st->n_first + 1;  // outer iterations that ran
+  if (inner_iters) *inner_iters += executed * inner;
+  *b.pcur = c0 ^ ((executed + 1) & 1);               // + the stopping launch's p update
+  *b.sdcur = *b.sdcur ^ ((executed * inner) & 1);
+  return TL_OK;
+}
+
+// Sweeps only (the polynomial preconditioner applied to r into z, from
+// scratch): tea_leaf_ppcg_init_sd + the inner steps + update_z, with the
+// partials of z.r left in slot P2.  Used to start an inner PPCG solve.
+template <typename T, int V>
+int run_ppcg_precondition(const LoopBufs<T> &b, const double *alphas, const double *betas,
+                          int inner, double theta) {
+  State &s = S();
+  const Geo G = b.G;
+  const int nb = abs_blocks(G);
+  const bool nt = use_nontemporal();
+  auto ksw = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
+  int sd = *b.sdcur;
+  const T theta_r = (T)(1.0 / theta);
+  for (int i = 1; i <= inner; ++i) {
+    hipLaunchKernelGGL(ksw, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
+                       i == inner ? 1 : 0, 0, theta_r, (T)alphas[i - 1], (T)betas[i - 1], b.diag,
+                       b.diag ? b.z : b.r, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r,
+                       b.r_store, b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2),
+                       (Ctl *)s.ctl);
+    sd = 1 - sd;
+  }
+  *b.sdcur = sd;
+  TL_HIP(hipGetLastError());
+  return TL_OK;
+}
+
+}  // namespace
+}  // namespace tl
