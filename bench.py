@@ -8,6 +8,9 @@ diffuse() loop body: timestep + tea_leaf() solve + field_summary, run by the
 native host with every field resident in HBM.
 
 Prints one JSON line.  `value` is seconds per timestep (lower is better);
+`precision_comparison` times FP64 CG / PPCG against the FP32-inner,
+FP64-refined forms at a relative residual of 1e-10 on the inverse-density
+deck;
 `roofline` uses the algorithmic bytes of the CG calc_w kernel (48 B per cell:
 read r, p, Kx, Ky; write p, w) over its HIP-event-timed launches;
 `cpu_baseline` times the reference Fortran/OpenMP build (oracle/_ref) on the
@@ -75,6 +78,91 @@ def gpu_run(cells, solver, steps, warmup, workdir):
     return elapsed, iters, prof
 
 
+def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
+    """Run `warmup` + `steps` timesteps of a deck through the native host;
+    returns the timed seconds, the per-step results of the timed steps and
+    the final u (interior) when asked."""
+    import ctypes as C
+    import numpy as np
+    from tealeaf_ref_amd import _lib
+    from tealeaf_ref_amd.host import TeaHost
+    deck_path = os.path.join(workdir, "tea.in")
+    with open(deck_path, "w") as f:
+        f.write(deck)
+    h = TeaHost(deck_path, os.path.join(workdir, "tea.out"))
+    for _ in range(warmup):
+        h.step()
+    _lib.call("tl_synchronize")
+    rows = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.step()
+        rows.append(h.last())
+    _lib.call("tl_synchronize")
+    elapsed = time.perf_counter() - t0
+    u = None
+    if want_u:
+        m = re.search(r"x_cells=(\d+)", deck)
+        n = int(m.group(1))
+        u = np.zeros((n + 2, n + 2))
+        _lib.call("tl_download", _lib.FIELD["u"], u.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        u = u[1:-1, 1:-1].copy()
+    h.close()
+    return elapsed, rows, u
+
+
+def precision_comparison(cells, steps, warmup, inner_steps=10):
+    """FP64 CG, FP64 PPCG and the FP32-inner / FP64-refined CG and PPCG on the
+    inverse-density deck, each to a relative residual of 1e-10 (FP64: the
+    reference's recurrence test plus tl_check_result for the true residual;
+    mixed: the true residual).  Errors of u and of the summary are relative
+    to the FP64 CG result."""
+    from tealeaf_ref_amd.deckgen import bench_deck
+    import numpy as np
+    runs = {}
+    base = dict(coefficient="inverse_density", eps=1e-10, max_iters=20000,
+                end_step=steps + warmup, summary_frequency=1, check_result=True)
+    plan = [("fp64_cg", "cg", ()), ("fp64_ppcg", "ppcg", ()),
+            ("mixed_cg", "cg", ("tl_mixed_precision",)),
+            ("mixed_ppcg", "ppcg", ("tl_mixed_precision",))]
+    ref_u = ref_sum = None
+    for name, solver, extra in plan:
+        kw = dict(base, extra=extra)
+        if solver == "ppcg":
+            kw["inner_steps"] = inner_steps
+        deck = bench_deck(cells, solver=solver, **kw)
+        with tempfile.TemporaryDirectory() as wd:
+            elapsed, rows, u = run_deck_timed(deck, steps, warmup, wd, want_u=True)
+        last = rows[-1]
+        rec = {"s_per_step": elapsed / steps,
+               "iterations_per_step": sum(r["iterations"] for r in rows) / steps,
+               "true_rel_residual": max(
+                   r["exact_error"] if not extra else r["conduction_error"] for r in rows)}
+        if solver == "ppcg":
+            rec["sweeps_per_step"] = sum(r["ppcg_inner"] for r in rows) / steps
+        if extra:
+            rec["refinements_per_step"] = sum(r["refinements"] for r in rows) / steps
+        summ = [last["volume"], last["mass"], last["energy"], last["temperature"]]
+        if ref_u is None:
+            ref_u, ref_sum = u, summ
+        else:
+            rec["u_rel_err_vs_fp64_cg"] = float(np.max(np.abs(u - ref_u)) / np.max(np.abs(ref_u)))
+            rec["summary_rel_err_vs_fp64_cg"] = max(
+                abs(a - b) / abs(b) for a, b in zip(summ, ref_sum) if b != 0.0)
+        log(f"[bench] {cells}^2 {name}: {rec}")
+        runs[name] = rec
+    out = {"workload": f"{cells}x{cells}, seeded deck (3 rectangles + 2 circles, seed 2), "
+                       f"tl_coefficient_inverse_density, relative residual 1e-10, "
+                       f"PPCG {inner_steps} inner steps, {steps} timed steps after {warmup}",
+           "runs": runs}
+    fp64 = runs["fp64_cg"]["s_per_step"]
+    out["speedup_mixed_ppcg_vs_fp64_cg"] = fp64 / runs["mixed_ppcg"]["s_per_step"]
+    out["speedup_mixed_cg_vs_fp64_cg"] = fp64 / runs["mixed_cg"]["s_per_step"]
+    out["speedup_mixed_ppcg_vs_fp64_ppcg"] = (runs["fp64_ppcg"]["s_per_step"]
+                                              / runs["mixed_ppcg"]["s_per_step"])
+    return out
+
+
 def cpu_baseline(cells, solver, gpu_iters_per_step, cap):
     """Reference build (oracle/_ref) on the host cores: one timestep capped at
     `cap` iterations, profiler_on -> Time Per It; falls back to the C oracle
@@ -126,6 +214,11 @@ def main():
     ap.add_argument("--solver", default="cg", choices=["cg"])
     ap.add_argument("--cpu-cap", type=int, default=100, help="iterations of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-sizes", default="4096",
+                    help="comma-separated mesh sizes of the FP64 vs FP32-refined comparison "
+                         "('' to skip)")
+    ap.add_argument("--mixed-steps", type=int, default=2)
+    ap.add_argument("--mixed-warmup", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver keeps the whole mesh as one chunk "
@@ -171,6 +264,10 @@ def main():
                      "calc_ur_achieved": ur_gbs,
                      "calc_ur_avg_launch_ms": ms_ur / n_ur if n_ur else None},
     }
+    sizes = [int(x) for x in args.mixed_sizes.split(",") if x.strip()]
+    if sizes:
+        out["precision_comparison"] = [
+            precision_comparison(n, args.mixed_steps, args.mixed_warmup) for n in sizes]
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap)

@@ -203,12 +203,53 @@ int tl_jacobi_loop(tl_loop_state *st);
 int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int step_first,
                  int count, int want_norm, double *norm);
 
+/* ---- mixed precision: FP32 inner solves under FP64 iterative refinement ---
+ * No counterpart in the reference: it replaces the whole iteration loop of
+ * tea_leaf() (tea_solve.f90:152-414) for a deck that asks for it.  Solves
+ * A u = u0 for the current step after tl_common_init, starting from the u
+ * that tl_common_init set (u = u0).  Each refinement computes the true
+ * residual r = u0 - A u in FP64, solves A d = r with FP32 fields and stencil
+ * (CG, or PPCG with the Chebyshev polynomial preconditioner of the
+ * reference) whose dot products, alpha/beta and eigenvalue bounds are FP64,
+ * and adds u += d in FP64.  It stops when ||u0 - A u|| <= rel_tol times the
+ * residual of the starting u (a true residual, unlike the recurrence
+ * residual of tea_solve.f90:402-410).  Requires tl_preconditioner_type none
+ * and a chunk with all four faces external. */
+#define TL_MIXED_CG 1
+#define TL_MIXED_PPCG 2
+
+typedef struct {
+  /* in */
+  int solver;             /* TL_MIXED_CG or TL_MIXED_PPCG                        */
+  int max_outer;          /* refinement limit                                    */
+  int max_inner;          /* iteration limit of one inner solve                  */
+  int inner_steps;        /* PPCG polynomial degree (tl_ppcg_inner_steps)        */
+  int presteps;           /* FP32 CG iterations that estimate the eigenvalues   */
+  double rel_tol;         /* true relative residual to reach                     */
+  double inner_tol;       /* residual reduction asked of an inner solve; one
+                           * that needs less to reach rel_tol is asked for a
+                           * quarter of what remains                             */
+  /* in/out: PPCG eigenvalue bounds; > 0 on entry reuses them (the matrix is
+   * the same every step), else they are estimated and returned */
+  double eigmin, eigmax;
+  /* out */
+  int outer;              /* refinements done                                    */
+  int inner_iters;        /* CG or PPCG outer iterations summed over inner solves */
+  int sweeps;             /* PPCG polynomial sweeps summed                       */
+  int reason;             /* TL_LOOP_CONVERGED or TL_LOOP_MAXITERS               */
+  double initial_residual;/* ||u0 - A u|| at entry                               */
+  double final_residual;  /* ||u0 - A u|| on exit (FP64)                         */
+} tl_mixed_state;
+
+int tl_mixed_solve(tl_mixed_state *st);
+
 /* ---- timing of the dominant kernels (HIP events on the library stream) --- */
 /* Enables per-launch HIP-event timing of the solver kernels. */
 int tl_profile_enable(int on);
 /* Returns launch count and total milliseconds for a kernel class:
  * 0 = CG calc_w, 1 = CG calc_ur, 2 = PPCG inner sweep, 3 = Chebyshev iterate,
- * 4 = Jacobi, 5 = PPCG calc_w, 6 = PPCG calc_ur. */
+ * 4 = Jacobi, 5 = PPCG calc_w, 6 = PPCG calc_ur, 7 = mixed-precision FP64
+ * refinement (u += d, true residual). */
 int tl_profile_get(int kernel_class, long *launches, double *total_ms);
 int tl_profile_reset(void);
 

@@ -20,8 +20,11 @@ FIELD = {"density": 1, "energy0": 2, "energy1": 3, "u": 4, "p": 5, "sd": 6, "r":
          "kx": 9, "ky": 10, "di": 11, "u0": 12, "w": 13, "r_store": 14, "mi": 15,
          "utemp": 16, "rtemp": 17, "volume": 18, "cp": 19, "bfp": 20}
 
+MIXED_CG, MIXED_PPCG = 1, 2
+
 # kernel classes of tl_profile_get
-PROF_CG_W, PROF_CG_UR, PROF_PPCG_SWEEP, PROF_CHEBY, PROF_JACOBI, PROF_PPCG_W, PROF_PPCG_UR = range(7)
+(PROF_CG_W, PROF_CG_UR, PROF_PPCG_SWEEP, PROF_CHEBY, PROF_JACOBI, PROF_PPCG_W, PROF_PPCG_UR,
+ PROF_MIXED_REFINE) = range(8)
 
 
 class TeaLeafError(RuntimeError):
@@ -33,6 +36,15 @@ class LoopState(C.Structure):
                 ("check_switch", C.c_int), ("epslim", C.c_double), ("presteps", C.c_int),
                 ("rro", C.c_double), ("old_error", C.c_double), ("n_end", C.c_int),
                 ("reason", C.c_int), ("error", C.c_double)]
+
+
+class MixedState(C.Structure):
+    _fields_ = [("solver", C.c_int), ("max_outer", C.c_int), ("max_inner", C.c_int),
+                ("inner_steps", C.c_int), ("presteps", C.c_int), ("rel_tol", C.c_double),
+                ("inner_tol", C.c_double), ("eigmin", C.c_double), ("eigmax", C.c_double),
+                ("outer", C.c_int), ("inner_iters", C.c_int), ("sweeps", C.c_int),
+                ("reason", C.c_int), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double)]
 
 
 _lib = None
@@ -64,6 +76,7 @@ def lib():
             "tl_ppcg_loop": [P(LoopState), PD, PD, I, D, P(I)],
             "tl_jacobi_loop": [P(LoopState)],
             "tl_cheby_run": [PD, PD, I, I, I, PD],
+            "tl_mixed_solve": [P(MixedState)],
             "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
             "tl_profile_reset": [],
         }
@@ -134,3 +147,12 @@ class Device:
     def common_init(self, rx, ry, zero_boundary=(1, 1, 1, 1)):
         zb = (C.c_int * 4)(*zero_boundary)
         call("tl_common_init", rx, ry, zb)
+
+    def mixed_solve(self, solver=MIXED_PPCG, rel_tol=1e-10, inner_tol=1e-5, max_outer=50,
+                    max_inner=20000, inner_steps=10, presteps=30, eigmin=0.0, eigmax=0.0):
+        """FP32-inner / FP64-refined solve of the current step (tl_mixed_solve)."""
+        st = MixedState(solver=solver, max_outer=max_outer, max_inner=max_inner,
+                        inner_steps=inner_steps, presteps=presteps, rel_tol=rel_tol,
+                        inner_tol=inner_tol, eigmin=eigmin, eigmax=eigmax)
+        call("tl_mixed_solve", C.byref(st))
+        return st

@@ -70,6 +70,12 @@ struct State {
   double *hist = nullptr;      // loop history: rro, alpha, beta (3 x hist_len)
   long hist_len = 0;
 
+  // FP32 fields of the mixed-precision solver (allocated on first use):
+  // rows of pitch32 floats, cell (1,1) at f32 + k*field32 + pitch32 + PADL32
+  float *f32 = nullptr;
+  long pitch32 = 0;
+  size_t field32 = 0;          // floats per field
+
   // profiling
   bool prof = false;
   struct Ev { hipEvent_t a, b; int cls; };

@@ -100,6 +100,13 @@ namespace {
 
 using dev::Geo;
 
+// FP32 fields belong to the mixed-precision solver, which applies the
+// operator in difference form; FP64 fields use the reference's Di form.
+template <typename T>
+constexpr bool diff_form() {
+  return sizeof(T) == 4;
+}
+
 // Scalar head of the first kernel of an iteration: beta of iteration n-1,
 // exit test (tea_solve.f90:402-410) and, for CG, the switch test (:167).
 struct Head {
@@ -189,7 +196,7 @@ __global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int t
       const long i = (long)row * P + col;
       return upd ? src[i] + beta * pin[i] : pin[i];
     };
-    dev::walkv<NT, T, V>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
+    dev::walkv<NT, T, V, diff_form<T>()>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
       if (tt.c < G.nx) {
         dev::stv<NT, T, V>(w + (long)y * P, tt.c, G.nx, wv);
         dev::stv<NT, T, V>(pout + (long)y * P, tt.c, G.nx, v);
@@ -306,10 +313,10 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
     T *__restrict__ rtemp, T *__restrict__ utemp, const T *__restrict__ r,
     const T *__restrict__ r_store, T *__restrict__ z, const T *__restrict__ Mi,
     const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry,
-    double *__restrict__ part_rrn, double *__restrict__ part_zr, Ctl *ctl) {
+    double *__restrict__ part_rrn, double *__restrict__ part_zr, const Ctl *ctl) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
-  if (ctl->stop) return;
+  if (ctl && ctl->stop) return;  // no control block: sweeps outside a loop
   const long P = G.pitch;
   double arrn = 0.0, azr = 0.0;
   dev::TileIter it(G.tiles, G.blocks);
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
       const long i = (long)row * P + col;
       return first_sweep ? src[i] * theta_r : sdin[i];
     };
-    dev::walkv<NT, T, V>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
+    dev::walkv<NT, T, V, diff_form<T>()>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
       if (tt.c >= G.nx) return;
       const long i = (long)y * P + tt.c;
       const VT rv = (first_sweep || last_sweep) ? dev::ldv<NT, T, V>(r + i) : VT(0);
@@ -527,7 +534,7 @@ int run_ppcg_precondition(const LoopBufs<T> &b, const double *alphas, const doub
                        i == inner ? 1 : 0, 0, theta_r, (T)alphas[i - 1], (T)betas[i - 1], b.diag,
                        b.diag ? b.z : b.r, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r,
                        b.r_store, b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2),
-                       (Ctl *)s.ctl);
+                       (const Ctl *)nullptr);
     sd = 1 - sd;
   }
   *b.sdcur = sd;

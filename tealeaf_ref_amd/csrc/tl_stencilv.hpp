@@ -53,6 +53,16 @@ __device__ inline T apply_t(T c, T l, T r, T d, T u, T kxl, T kxr, T kyd, T kyu,
   return di * c - ry * (kyu * u + kyd * d) - rx * (kxr * r + kxl * l);
 }
 
+// The same operator in difference form,
+//   (A x)_c = x_c + rx (Kl (x_c - x_l) + Kr (x_c - x_r)) + ry (Kd (x_c - x_d) + Ku (x_c - x_u)),
+// used by the mixed-precision solver: the differences of neighbouring values
+// are formed first, so rounding is relative to the fluxes instead of to
+// Di x_c (no loss proportional to the condition number for smooth x).
+template <typename T>
+__device__ inline T apply_diff(T c, T l, T r, T d, T u, T kxl, T kxr, T kyd, T kyu, T rx, T ry) {
+  return c + rx * (kxl * (c - l) + kxr * (c - r)) + ry * (kyd * (c - d) + kyu * (c - u));
+}
+
 template <int V>
 struct TileV {
   int x0;  // first column of this wave
@@ -73,10 +83,12 @@ __device__ inline TileV<V> tilev_of(const Geo &g, long t) {
 }
 
 // ldv_op(col, row) -> V-vector of operand values; ld1_op(col, row) -> one
-// value (only for the two cells outside the wave).  body(y, v, w) gets the
+// value (only for the two cells outside the wave).  DIFF selects the
+// difference form of the operator.  body(y, v, w) gets the
 // operand vector and A applied to it.  Lanes wholly outside [0, nx) run the
 // loop too (their values feed the shuffles) but must not store.
-template <bool NT, typename T, int V, typename LDV, typename LD1, typename BODY>
+template <bool NT, typename T, int V, bool DIFF = false, typename LDV, typename LD1,
+          typename BODY>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
                              BODY body) {
@@ -115,7 +127,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
       else r = (col + 1 < nx) ? right : vc[e];
       const T kl = kx[e];
       const T kr = (e + 1 < V) ? kx[e + 1] : kxr;
-      w[e] = apply_t<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry);
+      w[e] = DIFF ? apply_diff<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry)
+                  : apply_t<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry);
     }
     body(y, vc, w);
     vd = vc;

@@ -63,11 +63,11 @@ class TeaHost:
         return host_lib().tea_host_run()
 
     def last(self):
-        out = (C.c_double * 13)()
+        out = (C.c_double * 14)()
         host_lib().tea_host_last(out)
         keys = ("step", "time", "iterations", "ppcg_inner", "conduction_error", "exact_error",
                 "solve_time", "switch_n", "summary_step", "volume", "mass", "energy",
-                "temperature")
+                "temperature", "refinements")
         return dict(zip(keys, list(out)))
 
     def close(self):

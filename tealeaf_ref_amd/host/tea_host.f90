@@ -45,6 +45,11 @@ MODULE tea_host
   INTEGER :: halo_depth, test_problem, number_of_states
   LOGICAL :: tl_check_result, reflective_boundary, profiler_on, verbose_on
   LOGICAL :: tl_use_cg, tl_use_chebyshev, tl_use_ppcg, tl_use_jacobi
+  ! mixed precision (no reference counterpart): FP32 inner solves under FP64
+  ! iterative refinement, stopped on the true residual
+  LOGICAL :: tl_mixed_precision
+  REAL(KIND=8) :: tl_mixed_inner_eps
+  INTEGER :: tl_mixed_max_refinements
   TYPE(state_type), ALLOCATABLE :: states(:)
 
   ! ---- run state ---------------------------------------------------------------
@@ -57,9 +62,12 @@ MODULE tea_host
   ! SAVEd solver state of tea_solve.f90:55,58
   REAL(KIND=8) :: eigmin = 0.0_8, eigmax = 0.0_8, theta = 0.0_8, cn = 0.0_8
   LOGICAL :: first = .TRUE.
+  ! eigenvalue bounds of the mixed-precision PPCG, kept across steps
+  REAL(KIND=8) :: mixed_eigmin = 0.0_8, mixed_eigmax = 0.0_8
   ! results of the last step, read through tea_host_last()
   INTEGER :: last_iterations = 0, last_ppcg_inner = 0, last_switch = 0, last_est_itc = 0
   REAL(KIND=8) :: last_cond_error = 0.0_8, last_exact_error = 0.0_8, last_solve_time = 0.0_8
+  INTEGER :: last_refinements = 0
   REAL(KIND=8) :: last_summary(5) = 0.0_8
   INTEGER :: last_summary_step = -1
 
@@ -273,6 +281,9 @@ CONTAINS
     tl_use_chebyshev = .FALSE.; tl_use_cg = .FALSE.; tl_use_ppcg = .FALSE.; tl_use_jacobi = .TRUE.
     verbose_on = .FALSE.
     halo_depth = 1
+    tl_mixed_precision = .FALSE.
+    tl_mixed_inner_eps = -1.0_8          ! default by solver, set below
+    tl_mixed_max_refinements = 50
 
     WRITE(g_out, *) 'Reading input file'
     WRITE(g_out, *)
@@ -347,6 +358,15 @@ CONTAINS
           WRITE(g_out, "(1x,a25,e12.4)") 'tl_ch_cg_epslim', tl_ch_cg_epslim
         CASE ('tl_check_result')
           tl_check_result = .TRUE.
+        CASE ('tl_mixed_precision')
+          tl_mixed_precision = .TRUE.
+          WRITE(g_out, "(1x,a25)") 'tl_mixed_precision'
+        CASE ('tl_mixed_inner_eps')
+          tl_mixed_inner_eps = get_rval(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,e12.4)") 'tl_mixed_inner_eps', tl_mixed_inner_eps
+        CASE ('tl_mixed_max_refinements')
+          tl_mixed_max_refinements = get_ival(deck_getword(.TRUE.))
+          WRITE(g_out, "(1x,a25,i12)") 'tl_mixed_max_refinements', tl_mixed_max_refinements
         CASE ('tl_preconditioner_type')
           DO
             v = deck_getword(.FALSE.)
@@ -455,6 +475,15 @@ CONTAINS
     ENDIF
     IF (halo_depth > 1 .AND. tl_preconditioner_type == TL_PREC_JAC_BLOCK) &
       CALL report_error('read_input', 'Unable to use nonstandard halo depth with block jacobi preconditioner')
+    ! measured at 4096^2: FP32 CG restarts cost more than they save below 1e-4,
+    ! PPCG is best at 1e-5
+    IF (tl_mixed_inner_eps <= 0.0_8) tl_mixed_inner_eps = MERGE(1.0e-5_8, 1.0e-4_8, tl_use_ppcg)
+    IF (tl_mixed_precision .AND. .NOT. (tl_use_cg .OR. tl_use_ppcg)) &
+      CALL report_error('read_input', 'tl_mixed_precision needs tl_use_cg or tl_use_ppcg')
+    IF (tl_mixed_precision .AND. tl_preconditioner_type /= TL_PREC_NONE) &
+      CALL report_error('read_input', 'tl_mixed_precision needs tl_preconditioner_type none')
+    IF (tl_mixed_precision .AND. tl_use_ppcg .AND. tl_ppcg_inner_steps < 1) &
+      CALL report_error('read_input', 'tl_mixed_precision with PPCG needs tl_ppcg_inner_steps >= 1')
 
     WRITE(g_out, "(1x,a25,i12)") 'tiles per task ', 1
     WRITE(g_out, *)
@@ -768,7 +797,13 @@ CONTAINS
     initial_residual = SQRT(ABS(initial_residual))
     IF (verbose_on) WRITE(g_out, *) "Initial residual ", initial_residual
 
-    IF (tl_use_cg .OR. tl_use_chebyshev .OR. tl_use_ppcg) THEN
+    last_refinements = 0
+    IF (tl_mixed_precision) THEN
+      ! the whole iteration loop runs in the library (tl_mixed_solve)
+      IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init)
+      t_solve = timer()
+      CALL mixed_solve(initial_residual, error, n, ppcg_inner_iters, exited)
+    ELSE IF (tl_use_cg .OR. tl_use_chebyshev .OR. tl_use_ppcg) THEN
       CALL hip_check(tl_cg_init(rro), 'tea_leaf_cg_init')
       fields = 0
       fields(FIELD_U) = 1
@@ -781,12 +816,13 @@ CONTAINS
       fields(FIELD_U) = 1
     ENDIF
 
-    IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init)
-    t_solve = timer()
-
-    n = 1
-    exited = .FALSE.
-    outer: DO WHILE (n <= max_iters)
+    IF (.NOT. tl_mixed_precision) THEN
+      IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init)
+      t_solve = timer()
+      n = 1
+      exited = .FALSE.
+    ENDIF
+    outer: DO WHILE (n <= max_iters .AND. .NOT. tl_mixed_precision)
 
       IF (.NOT. ch_switch_check) THEN             ! tea_solve.f90:160-174
         IF (cheby_calc_steps > 0) THEN
@@ -967,7 +1003,7 @@ CONTAINS
         EXIT outer
       ENDIF
     ENDDO outer
-    IF (.NOT. exited) n = max_iters + 1
+    IF (.NOT. exited .AND. .NOT. tl_mixed_precision) n = max_iters + 1
 
     IF (tl_check_result) THEN                        ! :416-431
       fields = 0
@@ -1013,6 +1049,40 @@ CONTAINS
     ENDIF
     DEALLOCATE(cg_alphas, cg_betas, ch_alphas, ch_betas)
   END SUBROUTINE tea_leaf
+
+  !> Mixed-precision solve of the step (tl_mixed_solve): FP32 CG or PPCG
+  !> inner solves under FP64 refinement until the true residual has dropped
+  !> by eps.  On return n - 1 is the number of inner (CG or PPCG outer)
+  !> iterations, error the final true residual and sweeps the PPCG
+  !> polynomial sweeps.
+  SUBROUTINE mixed_solve(initial_residual, error, n, sweeps, converged)
+    REAL(KIND=8), INTENT(IN) :: initial_residual
+    REAL(KIND=8), INTENT(OUT) :: error
+    INTEGER, INTENT(OUT) :: n, sweeps
+    LOGICAL, INTENT(OUT) :: converged
+    TYPE(tl_mixed_state) :: ms
+    ms%solver = MERGE(TL_MIXED_PPCG, TL_MIXED_CG, tl_use_ppcg)
+    ms%max_outer = INT(tl_mixed_max_refinements, c_int)
+    ms%max_inner = INT(max_iters, c_int)
+    ms%inner_steps = INT(MAX(tl_ppcg_inner_steps, 1), c_int)
+    ms%presteps = INT(MAX(tl_ch_cg_presteps, 2), c_int)
+    ms%rel_tol = eps
+    ms%inner_tol = tl_mixed_inner_eps
+    ms%eigmin = mixed_eigmin
+    ms%eigmax = mixed_eigmax
+    CALL hip_check(tl_mixed_solve(ms), 'tea_leaf_mixed')
+    mixed_eigmin = ms%eigmin
+    mixed_eigmax = ms%eigmax
+    error = ms%final_residual
+    n = ms%inner_iters + 1
+    sweeps = ms%sweeps
+    converged = ms%reason == TL_LOOP_CONVERGED
+    last_refinements = ms%outer
+    WRITE(g_out, "('Mixed precision refinements',i6,' initial residual',e14.6,' true residual',e14.6)") &
+      ms%outer, initial_residual, ms%final_residual
+    IF (tl_use_ppcg) WRITE(g_out, "('Eigen min',e14.6,' Eigen max',e14.6)") ms%eigmin, ms%eigmax
+    IF (.NOT. converged) WRITE(g_out, "(a)") ' Mixed precision solve stopped at the refinement limit'
+  END SUBROUTINE mixed_solve
 
   !> tea_leaf_ppcg.f90:211-431 on one chunk
   SUBROUTINE run_ppcg_inner_steps(ca, cb, inner)
@@ -1218,11 +1288,12 @@ CONTAINS
     tea_host_run = INT(step, c_int)
   END FUNCTION tea_host_run
 
-  !> Results of the last step: out(1:13) = step, time, iterations,
+  !> Results of the last step: out(1:14) = step, time, iterations,
   !> ppcg inner iterations, conduction error, exact error, solve seconds,
-  !> switch iteration, summary step, vol, mass, ie, temp.
+  !> switch iteration, summary step, vol, mass, ie, temp, mixed-precision
+  !> refinements.
   INTEGER(c_int) FUNCTION tea_host_last(out) BIND(C, name="tea_host_last")
-    REAL(c_double), INTENT(OUT) :: out(13)
+    REAL(c_double), INTENT(OUT) :: out(14)
     out(1) = step
     out(2) = time
     out(3) = last_iterations
@@ -1236,6 +1307,7 @@ CONTAINS
     out(11) = last_summary(2)
     out(12) = last_summary(4)
     out(13) = last_summary(5)
+    out(14) = last_refinements
     tea_host_last = 0
   END FUNCTION tea_host_last
 

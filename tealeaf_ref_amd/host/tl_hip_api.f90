@@ -25,6 +25,15 @@ MODULE tl_hip_api
     REAL(c_double) :: error
   END TYPE tl_loop_state
 
+  INTEGER(c_int), PARAMETER :: TL_MIXED_CG = 1, TL_MIXED_PPCG = 2
+
+  TYPE, BIND(C) :: tl_mixed_state
+    INTEGER(c_int) :: solver, max_outer, max_inner, inner_steps, presteps
+    REAL(c_double) :: rel_tol, inner_tol, eigmin, eigmax
+    INTEGER(c_int) :: outer, inner_iters, sweeps, reason
+    REAL(c_double) :: initial_residual, final_residual
+  END TYPE tl_mixed_state
+
   INTERFACE
     INTEGER(c_int) FUNCTION tl_init(nx, ny, halo, device) BIND(C, name="tl_init")
       IMPORT :: c_int
@@ -132,6 +141,10 @@ MODULE tl_hip_api
       INTEGER(c_int), VALUE :: inner
       REAL(c_double), VALUE :: theta
       INTEGER(c_int), INTENT(INOUT) :: inner_iters
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_mixed_solve(st) BIND(C, name="tl_mixed_solve")
+      IMPORT :: c_int, tl_mixed_state
+      TYPE(tl_mixed_state), INTENT(INOUT) :: st
     END FUNCTION
     INTEGER(c_int) FUNCTION tl_jacobi_loop(st) BIND(C, name="tl_jacobi_loop")
       IMPORT :: c_int, tl_loop_state
