@@ -205,6 +205,31 @@ def cpu_baseline(cells, solver, gpu_iters_per_step, cap):
             "cell_iterations_per_s": cells * cells / tpi}
 
 
+def profile_run(cells, iters, copies=4):
+    """Short run for hardware-counter profiles: one 4096^2 CG timestep capped
+    at `iters` iterations, then `copies` launches of the plain copy kernel
+    (tl_ppcg_store_r: r_store = r over the interior, 8 B read + 8 B written
+    per cell), whose counters calibrate FETCH_SIZE / WRITE_SIZE."""
+    from tealeaf_ref_amd import _lib
+    from tealeaf_ref_amd.deckgen import bench_deck
+    from tealeaf_ref_amd.host import TeaHost
+    deck = bench_deck(cells, solver="cg", eps=1e-15, max_iters=iters, end_step=1,
+                      summary_frequency=1)
+    with tempfile.TemporaryDirectory() as wd:
+        path = os.path.join(wd, "tea.in")
+        with open(path, "w") as f:
+            f.write(deck)
+        h = TeaHost(path, os.path.join(wd, "tea.out"))
+        h.step()
+        for _ in range(copies):
+            _lib.call("tl_ppcg_store_r")
+        _lib.call("tl_synchronize")
+        it = h.last()["iterations"]
+        h.close()
+    print(json.dumps({"profile_run": True, "cells": cells * cells, "cg_iterations": it,
+                      "copy_launches": copies, "copy_bytes_each": 16 * cells * cells}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -217,6 +242,9 @@ def main():
     ap.add_argument("--mixed-sizes", default="4096",
                     help="comma-separated mesh sizes of the FP64 vs FP32-refined comparison "
                          "('' to skip)")
+    ap.add_argument("--profile-iters", type=int, default=0,
+                    help="counter-profiling mode: one CG step capped at this many iterations "
+                         "plus copy-kernel calibration launches; prints a short JSON line")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-warmup", type=int, default=1)
     args = ap.parse_args()
@@ -224,6 +252,10 @@ def main():
         print("bench.py runs on one GPU only: the solver keeps the whole mesh as one chunk "
               "on a single MI355X (multi-GPU decomposition is not built).", file=sys.stderr)
         sys.exit(2)
+
+    if args.profile_iters:
+        profile_run(args.cells, args.profile_iters)
+        return
 
     with tempfile.TemporaryDirectory() as wd:
         elapsed, iters, prof = gpu_run(args.cells, args.solver, args.steps, args.warmup, wd)

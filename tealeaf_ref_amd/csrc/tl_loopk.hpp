@@ -107,6 +107,34 @@ constexpr bool diff_form() {
   return sizeof(T) == 4;
 }
 
+// Row prefetch of the stencil walks (see walkv) per kernel.
+#ifndef TL_PF_W
+#define TL_PF_W 1
+#endif
+#ifndef TL_PF_SWEEP
+#define TL_PF_SWEEP 0
+#endif
+// Stores: written fields are re-read by the next kernel, so plain stores that
+// leave them in the 256 MB Infinity Cache beat streaming stores, except for
+// the FP64 PPCG sweeps, which write three fields per pass (measured at
+// 4096^2).  The CG update walks its rows in reverse so that it reads first
+// what the w = A p pass wrote last.
+#ifndef TL_ST_NT_LOOP
+#define TL_ST_NT_LOOP 0
+#endif
+#ifndef TL_ST_NT_SWEEP
+#define TL_ST_NT_SWEEP 1
+#endif
+#ifndef TL_UR_REV
+#define TL_UR_REV 1
+#endif
+#ifndef TL_LDSX_W
+#define TL_LDSX_W 1
+#endif
+#ifndef TL_LDSX_SWEEP
+#define TL_LDSX_SWEEP 1
+#endif
+
 // Scalar head of the first kernel of an iteration: beta of iteration n-1,
 // exit test (tea_solve.f90:402-410) and, for CG, the switch test (:167).
 struct Head {
@@ -183,7 +211,7 @@ __global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int t
         for (int y = tt.y0; y < tt.y1; ++y) {
           const long i = (long)y * P + tt.c;
           const VT v = dev::ldv<NT, T, V>(src + i) + beta * dev::ldv<NT, T, V>(pin + i);
-          dev::stv<NT, T, V>(pout + (long)y * P, tt.c, G.nx, v);
+          dev::stv<(NT && TL_ST_NT_LOOP), T, V>(pout + (long)y * P, tt.c, G.nx, v);
         }
       continue;
     }
@@ -196,10 +224,12 @@ __global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int t
       const long i = (long)row * P + col;
       return upd ? src[i] + beta * pin[i] : pin[i];
     };
-    dev::walkv<NT, T, V, diff_form<T>()>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
+    auto none = [](int) { return dev::NoExtra(); };
+    dev::walkv<NT, T, V, diff_form<T>(), TL_PF_W, TL_LDSX_W>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, none, [&](int y, VT v, VT wv, dev::NoExtra) {
       if (tt.c < G.nx) {
-        dev::stv<NT, T, V>(w + (long)y * P, tt.c, G.nx, wv);
-        dev::stv<NT, T, V>(pout + (long)y * P, tt.c, G.nx, v);
+        dev::stv<(NT && TL_ST_NT_LOOP), T, V>(w + (long)y * P, tt.c, G.nx, wv);
+        dev::stv<(NT && TL_ST_NT_LOOP), T, V>(pout + (long)y * P, tt.c, G.nx, v);
 #pragma unroll
         for (int e = 0; e < V; ++e)
           if (tt.c + e < G.nx) acc += (double)wv[e] * (double)v[e];
@@ -235,15 +265,16 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
   for (long t = it.t; t < it.t_end; ++t) {
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
     if (tt.c >= G.nx) continue;
-    for (int y = tt.y0; y < tt.y1; ++y) {
+    for (int k = tt.y0; k < tt.y1; ++k) {
+      const int y = TL_UR_REV ? tt.y1 - 1 - (k - tt.y0) : k;
       const long i = (long)y * P + tt.c;
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
       const VT rv = dev::ldv<NT, T, V>(r + i) - a * dev::ldv<NT, T, V>(w + i);
-      dev::stv<NT, T, V>(u + (long)y * P, tt.c, G.nx, uv);
-      dev::stv<NT, T, V>(r + (long)y * P, tt.c, G.nx, rv);
+      dev::stv<(NT && TL_ST_NT_LOOP), T, V>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::stv<(NT && TL_ST_NT_LOOP), T, V>(r + (long)y * P, tt.c, G.nx, rv);
       if (diag) {
         const VT zv = dev::ldv<NT, T, V>(Mi + i) * rv;
-        dev::stv<NT, T, V>(z + (long)y * P, tt.c, G.nx, zv);
+        dev::stv<(NT && TL_ST_NT_LOOP), T, V>(z + (long)y * P, tt.c, G.nx, zv);
 #pragma unroll
         for (int e = 0; e < V; ++e)
           if (tt.c + e < G.nx) acc += (double)rv[e] * (double)zv[e];
@@ -287,15 +318,16 @@ __global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double
   for (long t = it.t; t < it.t_end; ++t) {
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
     if (tt.c >= G.nx) continue;
-    for (int y = tt.y0; y < tt.y1; ++y) {
+    for (int k = tt.y0; k < tt.y1; ++k) {
+      const int y = TL_UR_REV ? tt.y1 - 1 - (k - tt.y0) : k;
       const long i = (long)y * P + tt.c;
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
       const VT ro = dev::ldv<NT, T, V>(r + i);
       const VT rv = ro - a * dev::ldv<NT, T, V>(w + i);
-      dev::stv<NT, T, V>(u + (long)y * P, tt.c, G.nx, uv);
-      dev::stv<NT, T, V>(r_store + (long)y * P, tt.c, G.nx, ro);
-      dev::stv<NT, T, V>(r + (long)y * P, tt.c, G.nx, rv);
-      if (diag) dev::stv<NT, T, V>(z + (long)y * P, tt.c, G.nx, dev::ldv<NT, T, V>(Mi + i) * rv);
+      dev::stv<(NT && TL_ST_NT_LOOP), T, V>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::stv<(NT && TL_ST_NT_LOOP), T, V>(r_store + (long)y * P, tt.c, G.nx, ro);
+      dev::stv<(NT && TL_ST_NT_LOOP), T, V>(r + (long)y * P, tt.c, G.nx, rv);
+      if (diag) dev::stv<(NT && TL_ST_NT_LOOP), T, V>(z + (long)y * P, tt.c, G.nx, dev::ldv<NT, T, V>(Mi + i) * rv);
     }
   }
 }
@@ -330,21 +362,36 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
       const long i = (long)row * P + col;
       return first_sweep ? src[i] * theta_r : sdin[i];
     };
-    dev::walkv<NT, T, V, diff_form<T>()>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VT v, VT wv) {
+    // per-row loads besides the operand: r, rtemp, utemp, r_store, Mi
+    struct Side {
+      VT rv, rt, ut, rs, mi;
+    };
+    const int cc = tt.c <= G.nx ? tt.c : (G.nx / V) * V;
+    auto side = [&](int row) -> Side {
+      const long i = (long)row * P + cc;
+      Side e;
+      e.rv = (first_sweep || last_sweep) ? dev::ldv<NT, T, V>(r + i) : VT(0);
+      e.rt = first_sweep ? VT(0) : dev::ldv<NT, T, V>(rtemp + i);
+      e.ut = first_sweep ? VT(0) : dev::ldv<NT, T, V>(utemp + i);
+      e.rs = (last_sweep && want_rrn) ? dev::ldv<NT, T, V>(r_store + i) : VT(0);
+      e.mi = diag ? dev::ldv<NT, T, V>(Mi + i) : VT(0);
+      return e;
+    };
+    dev::walkv<NT, T, V, diff_form<T>(), TL_PF_SWEEP, TL_LDSX_SWEEP>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, side, [&](int y, VT v, VT wv, const Side &e) {
       if (tt.c >= G.nx) return;
-      const long i = (long)y * P + tt.c;
-      const VT rv = (first_sweep || last_sweep) ? dev::ldv<NT, T, V>(r + i) : VT(0);
-      const VT rt0 = first_sweep ? rv : dev::ldv<NT, T, V>(rtemp + i);
-      const VT ut0 = first_sweep ? v : dev::ldv<NT, T, V>(utemp + i);
+      const VT rv = e.rv;
+      const VT rt0 = first_sweep ? rv : e.rt;
+      const VT ut0 = first_sweep ? v : e.ut;
       const VT rt = rt0 - wv;
-      const VT sv = diag ? alpha * v + (beta * dev::ldv<NT, T, V>(Mi + i)) * rt : alpha * v + beta * rt;
+      const VT sv = diag ? alpha * v + (beta * e.mi) * rt : alpha * v + beta * rt;
       const VT ut = ut0 + sv;
-      dev::stv<NT, T, V>(rtemp + (long)y * P, tt.c, G.nx, rt);
-      dev::stv<NT, T, V>(sdout + (long)y * P, tt.c, G.nx, sv);
-      dev::stv<NT, T, V>(utemp + (long)y * P, tt.c, G.nx, ut);
+      dev::stv<(NT && TL_ST_NT_SWEEP && sizeof(T) == 8), T, V>(rtemp + (long)y * P, tt.c, G.nx, rt);
+      dev::stv<(NT && TL_ST_NT_SWEEP && sizeof(T) == 8), T, V>(sdout + (long)y * P, tt.c, G.nx, sv);
+      dev::stv<(NT && TL_ST_NT_SWEEP && sizeof(T) == 8), T, V>(utemp + (long)y * P, tt.c, G.nx, ut);
       if (last_sweep) {
-        dev::stv<NT, T, V>(z + (long)y * P, tt.c, G.nx, ut);
-        const VT rs = want_rrn ? dev::ldv<NT, T, V>(r_store + i) : VT(0);
+        dev::stv<(NT && TL_ST_NT_SWEEP && sizeof(T) == 8), T, V>(z + (long)y * P, tt.c, G.nx, ut);
+        const VT rs = e.rs;
 #pragma unroll
         for (int e = 0; e < V; ++e)
           if (tt.c + e < G.nx) {

@@ -81,11 +81,16 @@ __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev
   }
 }
 
+// Columns per lane of the FP64 loop kernels (16-byte loads at 2).
+#ifndef TL_V64
+#define TL_V64 2
+#endif
+
 // The FP64 fields as one buffer set of the shared loop kernels.
 LoopBufs<double> fp64_bufs() {
   State &s = S();
   LoopBufs<double> b;
-  b.G = geo(2 * BLOCK);
+  b.G = geo(TL_V64 * BLOCK);
   b.u = s.u[s.ucur].p11;
   b.r = s.r.p11;
   b.w = s.w.p11;
@@ -218,7 +223,7 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
     return TL_OK;
   }
   LoopBufs<double> bb = fp64_bufs();
-  return run_cg_loop<double, 2>(bb, st, cg_alphas, cg_betas);
+  return run_cg_loop<double, TL_V64>(bb, st, cg_alphas, cg_betas);
 }
 
 extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const double *betas,
@@ -263,7 +268,7 @@ extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const doubl
   double rro_first;
   if ((rc = tl_ppcg_calc_zrnorm(&rro_first))) return rc;
   LoopBufs<double> bb = fp64_bufs();
-  return run_ppcg_loop<double, 2>(bb, st, alphas, betas, inner, theta, rro_first, nullptr,
+  return run_ppcg_loop<double, TL_V64>(bb, st, alphas, betas, inner, theta, rro_first, nullptr,
                                   inner_iters);
 }
 

@@ -121,13 +121,15 @@ __global__ __launch_bounds__(BLOCK) void k_mx_refine(Geo G, long pitch32,
       if (WITH_D) v += (double)d[(long)row * pitch32 + col];
       return v;
     };
-    dev::walkv<NT, double, 2, true>(G, tt, Kx, Ky, rx, ry, ldop, ld1, [&](int y, VD v, VD wv) {
+    const int cc = tt.c <= G.nx ? tt.c : (G.nx / 2) * 2;
+    auto side = [&](int row) -> VD { return dev::ldv<NT, double, 2>(u0 + (long)row * P + cc); };
+    dev::walkv<NT, double, 2, true, true>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, side, [&](int y, VD v, VD wv, VD u0v) {
       if (tt.c >= G.nx) return;
-      const long i = (long)y * P + tt.c;
-      const VD r = dev::ldv<NT, double, 2>(u0 + i) - wv;
+      const VD r = u0v - wv;
       const VF rf = __builtin_convertvector(r, VF);
-      if (WITH_D) dev::stv<NT, double, 2>(uout + (long)y * P, tt.c, G.nx, v);
-      dev::stv<NT, float, 2>(r32 + (long)y * pitch32, tt.c, G.nx, rf);
+      if (WITH_D) dev::stv<(NT && TL_ST_NT_LOOP), double, 2>(uout + (long)y * P, tt.c, G.nx, v);
+      dev::stv<(NT && TL_ST_NT_LOOP), float, 2>(r32 + (long)y * pitch32, tt.c, G.nx, rf);
       aa += r.x * r.x;
       ab += (double)rf.x * (double)rf.x;
       if (tt.c + 1 < G.nx) {

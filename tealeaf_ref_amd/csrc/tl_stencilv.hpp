@@ -20,10 +20,14 @@ struct Vec {
   typedef T type __attribute__((ext_vector_type(V)));
 };
 
+#ifndef TL_NT_LOADS
+#define TL_NT_LOADS 1
+#endif
+
 template <bool NT, typename T, int V>
 __device__ inline typename Vec<T, V>::type ldv(const T *p) {
   typedef typename Vec<T, V>::type VT;
-  if (NT) return __builtin_nontemporal_load((const VT *)p);
+  if (NT && TL_NT_LOADS) return __builtin_nontemporal_load((const VT *)p);
   return *(const VT *)p;
 }
 
@@ -82,40 +86,92 @@ __device__ inline TileV<V> tilev_of(const Geo &g, long t) {
   return tt;
 }
 
+// Per-row data a walk body needs besides the operand (none).
+struct NoExtra {};
+
 // ldv_op(col, row) -> V-vector of operand values; ld1_op(col, row) -> one
-// value (only for the two cells outside the wave).  DIFF selects the
-// difference form of the operator.  body(y, v, w) gets the
-// operand vector and A applied to it.  Lanes wholly outside [0, nx) run the
-// loop too (their values feed the shuffles) but must not store.
-template <bool NT, typename T, int V, bool DIFF = false, typename LDV, typename LD1,
-          typename BODY>
+// value (only for the two cells outside the wave); extra(row) -> the other
+// per-row loads of the body.  body(y, v, w, e) gets the operand vector, A
+// applied to it and extra(y).  DIFF selects the difference form of the
+// operator.  With PF every load of row y+1's step (operand row y+2, faces,
+// edge cells, extra) is issued before row y is computed, so one row of loads
+// is always in flight behind the arithmetic; the body itself must not load.
+// Lanes wholly outside [0, nx) run the loop too (their values feed the
+// shuffles) but must not store.
+// With LDSX the cells just outside a wave that belong to another wave of the
+// same block are exchanged through LDS (one barrier per row) instead of being
+// loaded again; only the block's outer edges are loaded.  All waves of the
+// block must walk the same rows.
+template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, typename LDV,
+          typename LD1, typename EXTRA, typename BODY>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
-                             BODY body) {
+                             EXTRA extra, BODY body) {
   typedef typename Vec<T, V>::type VT;
+  typedef decltype(extra(0)) E;
+  constexpr int NW = BLOCK / WAVE;
+  __shared__ T xch[2][NW][3];  // per row parity and wave: first value, last value, first Kx
   const long P = g.pitch;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int c = tt.c, nx = g.nx;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+  const int c = tt.c, nx = g.nx, ny = g.ny;
+  const int y0 = tt.y0, y1 = tt.y1;
+  if (y0 >= y1) return;
   // lanes past the mesh clamp their column so addresses stay in the row
   const int cl = c <= nx ? c : (nx / V) * V;
   const int xr = tt.x0 + WAVE * V;  // first column right of this wave
-  const bool need_left = lane == 0 && tt.x0 > 0;
-  const bool need_right = lane == WAVE - 1 && xr < nx;
-  const bool need_kxr = lane == WAVE - 1 && xr <= nx;
-  VT vd = ldv_op(cl, tt.y0 > 0 ? tt.y0 - 1 : tt.y0);
-  VT vc = ldv_op(cl, tt.y0);
-  VT kyd = ldv<NT, T, V>(Ky + (long)tt.y0 * P + cl);
-  for (int y = tt.y0; y < tt.y1; ++y) {
-    const bool last_row = y + 1 >= g.ny;
-    const VT vu = last_row ? vc : ldv_op(cl, y + 1);
-    const VT kyu = ldv<NT, T, V>(Ky + (long)(y + 1) * P + cl);
-    const VT kx = ldv<NT, T, V>(Kx + (long)y * P + cl);
+  const bool edge_l = !LDSX || wave == 0, edge_r = !LDSX || wave == NW - 1;
+  const bool need_left = lane == 0 && tt.x0 > 0 && edge_l;
+  const bool need_right = lane == WAVE - 1 && xr < nx && edge_r;
+  const bool need_kxr = lane == WAVE - 1 && xr <= nx && edge_r;
+  // state of row y: operand rows y-1, y, y+1 (mirrored at the mesh edge),
+  // faces Ky(y), Ky(y+1), Kx(y), the cells left and right of the wave,
+  // Kx right of the wave, and the extra data
+  VT vd = ldv_op(cl, y0 > 0 ? y0 - 1 : y0);
+  VT vc = ldv_op(cl, y0);
+  VT vu = y0 + 1 < ny ? ldv_op(cl, y0 + 1) : vc;
+  VT kyd = ldv<NT, T, V>(Ky + (long)y0 * P + cl);
+  VT kyu = ldv<NT, T, V>(Ky + (long)(y0 + 1) * P + cl);
+  VT kx = ldv<NT, T, V>(Kx + (long)y0 * P + cl);
+  T el = need_left ? ld1_op(tt.x0 - 1, y0) : T(0);
+  T er = need_right ? ld1_op(xr, y0) : T(0);
+  T ek = need_kxr ? Kx[(long)y0 * P + xr] : T(0);
+  E ex = extra(y0);
+  for (int y = y0; y < y1; ++y) {
+    const int yn = y + 1;
+    const bool more = yn < y1;
+    VT vu2 = vu, kyu2 = kyu, kx2 = kx;
+    T el2 = el, er2 = er, ek2 = ek;
+    E ex2 = ex;
+    auto load_next = [&]() {
+      vu2 = yn + 1 < ny ? ldv_op(cl, yn + 1) : vu;
+      kyu2 = ldv<NT, T, V>(Ky + (long)(yn + 1) * P + cl);
+      kx2 = ldv<NT, T, V>(Kx + (long)yn * P + cl);
+      if (need_left) el2 = ld1_op(tt.x0 - 1, yn);
+      if (need_right) er2 = ld1_op(xr, yn);
+      if (need_kxr) ek2 = Kx[(long)yn * P + xr];
+      ex2 = extra(yn);
+    };
+    if (PF && more) load_next();
     T left = __shfl_up(vc[V - 1], 1, WAVE);
     T right = __shfl_down(vc[0], 1, WAVE);
     T kxr = __shfl_down(kx[0], 1, WAVE);
-    if (need_left) left = ld1_op(tt.x0 - 1, y);
-    if (need_right) right = ld1_op(xr, y);
-    if (need_kxr) kxr = Kx[(long)y * P + xr];
+    if (LDSX) {
+      const int par = y & 1;
+      if (lane == 0) {
+        xch[par][wave][0] = vc[0];
+        xch[par][wave][2] = kx[0];
+      }
+      if (lane == WAVE - 1) xch[par][wave][1] = vc[V - 1];
+      __syncthreads();
+      if (lane == 0 && wave > 0) left = xch[par][wave - 1][1];
+      if (lane == WAVE - 1 && wave < NW - 1) {
+        right = xch[par][wave + 1][0];
+        kxr = xch[par][wave + 1][2];
+      }
+    }
+    if (need_left) left = el;
+    if (need_right) right = er;
+    if (need_kxr) kxr = ek;
     if (c == 0) left = vc[0];  // mirror at the left edge
     VT w;
 #pragma unroll
@@ -130,10 +186,18 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
       w[e] = DIFF ? apply_diff<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry)
                   : apply_t<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry);
     }
-    body(y, vc, w);
+    body(y, vc, w, ex);
+    if (!PF && more) load_next();
     vd = vc;
     vc = vu;
+    vu = vu2;
     kyd = kyu;
+    kyu = kyu2;
+    kx = kx2;
+    el = el2;
+    er = er2;
+    ek = ek2;
+    ex = ex2;
   }
 }
 
