@@ -243,6 +243,34 @@ typedef struct {
 
 int tl_mixed_solve(tl_mixed_state *st);
 
+/* ---- adjoint of the implicit step ----------------------------------------
+ * No counterpart in the reference: the building blocks of a reverse-mode
+ * gradient of a functional of u after N steps with respect to the per-cell
+ * log-density.  A step solves A(rho) u_n = b_n (b_1 = rho e_0, then
+ * b_n = u_{n-1}); A is symmetric, so the adjoint of a solve is another solve
+ * with the same operator (tl_cg_loop or tl_mixed_solve on the right-hand side
+ * placed in u0), and the dependence of A on rho sits in the face
+ * coefficients.  Snapshot slots are device arrays laid out like a field: the
+ * checkpoints, the adjoint seed and the gradient stay in HBM. */
+
+/* Reserve at least `count` snapshot slots (zeroed; kept until tl_finalize). */
+int tl_snapshot_reserve(int count);
+/* slot <- field, field <- slot (the whole allocation, halos included). */
+int tl_snapshot_save(int slot, int field_id);
+int tl_snapshot_load(int slot, int field_id);
+/* Every value of a slot, halos included, set to `value`. */
+int tl_snapshot_fill(int slot, double value);
+/* Host <-> slot copies, as tl_upload / tl_download. */
+int tl_snapshot_upload(int slot, const double *host, int host_halo);
+int tl_snapshot_download(int slot, double *host, int host_halo);
+/* grad += dJ/dlog(rho) of one step: for every cell c and each of its internal
+ * faces f (neighbour b), -r_f (lambda_c - lambda_b)(u_c - u_b) dK_f/dlog(rho_c)
+ * with r_f = rx or ry of the last tl_common_init and dK_f/dlog(rho_c) =
+ * -1/(2 rho_c) (tl_coefficient_density) or rho_c/2 (inverse density); plus
+ * lambda_c rho_c e0_c when e0_slot >= 0 (the first step's b_1 = rho e_0).
+ * lambda is read from a field, u_n, e_0 and grad from slots. */
+int tl_adjoint_face_grad(int lambda_field, int u_slot, int e0_slot, int grad_slot);
+
 /* ---- timing of the dominant kernels (HIP events on the library stream) --- */
 /* Enables per-launch HIP-event timing of the solver kernels. */
 int tl_profile_enable(int on);

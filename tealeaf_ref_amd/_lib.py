@@ -77,6 +77,9 @@ def lib():
             "tl_jacobi_loop": [P(LoopState)],
             "tl_cheby_run": [PD, PD, I, I, I, PD],
             "tl_mixed_solve": [P(MixedState)],
+            "tl_snapshot_reserve": [I], "tl_snapshot_save": [I, I], "tl_snapshot_load": [I, I],
+            "tl_snapshot_fill": [I, D], "tl_snapshot_upload": [I, PD, I],
+            "tl_snapshot_download": [I, PD, I], "tl_adjoint_face_grad": [I, I, I, I],
             "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
             "tl_profile_reset": [],
         }

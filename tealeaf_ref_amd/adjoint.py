@@ -1,0 +1,185 @@
+"""Gradient of N TeaLeaf timesteps with respect to the per-cell log-density.
+
+`TeaLeafSteps` runs the implicit heat-conduction step of the reference
+(tea_leaf.f90 -> tea_leaf_init_common, the CG solve of tea_solve.f90,
+tea_leaf_finalise) N times on one GPU through libtealeaf_hip.so, starting
+from a given energy field and a per-cell density, and returns u after the
+last solve.  `TeaLeafSteps.apply(theta)` is a torch.autograd.Function: its
+backward is the hand-written adjoint (tl_adjoint.hip): one solve with the
+same symmetric operator per step, plus the face-gradient kernel chained to
+log-density.  The forward keeps the energy every `checkpoint_every` steps in
+device snapshot slots; the backward recomputes each segment from its
+checkpoint and walks it in reverse.  Solves are FP64 CG, or the FP32-inner,
+FP64-refined solver (precision="mixed").
+
+The library holds one chunk per process: one TeaLeafSteps at a time.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+FIELDS = 11  # FIELD_* slots of tl_update_halo
+
+
+def _halo_flags(*names):
+    order = ("density", "energy0", "energy1", "u", "p", "sd", "r", "z", "kx", "ky", "di")
+    f = [0] * FIELDS
+    for n in names:
+        f[order.index(n)] = 1
+    return f
+
+
+def _pad(a):
+    """Interior (ny, nx) -> (ny+2, nx+2) with the edge values copied into the halo."""
+    return np.pad(np.asarray(a, dtype=np.float64), 1, mode="edge")
+
+
+class TeaLeafSteps:
+    """N implicit steps on an nx*ny chunk with a per-cell density.
+
+    energy0: initial energy1 field (ny, nx); rx, ry: dt/dx^2, dt/dy^2;
+    coefficient: _lib.CONDUCTIVITY or _lib.RECIP_CONDUCTIVITY; eps: relative
+    residual of every solve (recurrence residual for FP64 CG as in
+    tea_solve.f90:410, true residual for the mixed solver).
+    """
+
+    def __init__(self, energy0, rx, ry, steps, coefficient=L.CONDUCTIVITY, eps=1e-15,
+                 max_iters=20000, checkpoint_every=5, precision="fp64", reflective=False,
+                 device=0, mixed_inner_tol=1e-5):
+        e0 = np.asarray(energy0, dtype=np.float64)
+        self.ny, self.nx = e0.shape
+        self.rx, self.ry = float(rx), float(ry)
+        self.steps = int(steps)
+        self.k = max(1, int(checkpoint_every))
+        self.eps, self.max_iters = float(eps), int(max_iters)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.mixed_inner_tol = mixed_inner_tol
+        self.reflective = bool(reflective)
+        self.iterations = 0            # solver iterations of the last forward/backward
+        self.dev = L.Device(self.nx, self.ny, 1, device)
+        self.dev.set_options(coefficient, L.PREC_NONE, self.reflective)
+        self.dev.upload("energy1", _pad(e0))
+        nck = (self.steps + self.k - 1) // self.k
+        # slots: e_0, adjoint right-hand side, gradient, zeros, checkpoints, segment
+        self.E0, self.RHS, self.GRAD, self.ZERO = 0, 1, 2, 3
+        self.CK = 4
+        self.SEG = self.CK + nck
+        L.call("tl_snapshot_reserve", self.SEG + self.k)
+        L.call("tl_snapshot_save", self.E0, L.FIELD["energy1"])
+        L.call("tl_snapshot_fill", self.ZERO, 0.0)
+        self._forward_done = False
+
+    def close(self):
+        self.dev.close()
+
+    # -- one step ---------------------------------------------------------------
+    def _solve(self):
+        """Solve A u = u0 from the current u (tea_solve.f90:37-414, CG branch)."""
+        d = self.dev
+        if self.reflective:
+            d.update_halo(_halo_flags("u"), 1)
+        if self.precision == "mixed":
+            st = d.mixed_solve(solver=L.MIXED_CG, rel_tol=self.eps, inner_tol=self.mixed_inner_tol,
+                               max_inner=self.max_iters)
+            self.iterations += st.inner_iters
+            return
+        L.call("tl_calc_residual")
+        rr = d.scalar("tl_calc_2norm", 1)
+        if rr == 0.0:
+            return
+        rro = d.scalar("tl_cg_init")
+        st = L.LoopState(n_first=1, n_last=self.max_iters, eps_abs=self.eps * math.sqrt(abs(rr)),
+                         check_switch=0, epslim=0.0, presteps=0, rro=rro, old_error=rr)
+        L.call("tl_cg_loop", C.byref(st), None, None)
+        self.iterations += st.n_end
+
+    def _step(self):
+        d = self.dev
+        d.update_halo(_halo_flags("energy1", "density"), 1)
+        d.common_init(self.rx, self.ry)
+        self._solve()
+        L.call("tl_finalise")
+
+    # -- forward / backward --------------------------------------------------------
+    def forward(self, theta):
+        """u after `steps` steps for log-density theta (ny, nx); numpy in, numpy out."""
+        th = np.asarray(theta, dtype=np.float64)
+        if th.shape != (self.ny, self.nx):
+            raise ValueError(f"theta must have shape {(self.ny, self.nx)}")
+        self.dev.upload("density", _pad(np.exp(th)))
+        L.call("tl_snapshot_load", self.E0, L.FIELD["energy1"])
+        self.iterations = 0
+        for n in range(1, self.steps + 1):
+            if (n - 1) % self.k == 0:
+                L.call("tl_snapshot_save", self.CK + (n - 1) // self.k, L.FIELD["energy1"])
+            self._step()
+        self._forward_done = True
+        return self.dev.download("u")[1:-1, 1:-1].copy()
+
+    def backward(self, grad_u):
+        """dJ/dtheta from dJ/du_N (ny, nx) for the theta of the last forward."""
+        if not self._forward_done:
+            raise RuntimeError("backward needs a forward of the same TeaLeafSteps first")
+        g = np.asarray(grad_u, dtype=np.float64)
+        if g.shape != (self.ny, self.nx):
+            raise ValueError(f"grad_u must have shape {(self.ny, self.nx)}")
+        host = np.zeros((self.ny + 2, self.nx + 2))
+        host[1:-1, 1:-1] = g
+        L.call("tl_snapshot_upload", self.RHS, host.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        L.call("tl_snapshot_fill", self.GRAD, 0.0)
+        self.iterations = 0
+        nseg = (self.steps + self.k - 1) // self.k
+        for seg in range(nseg - 1, -1, -1):
+            a, b = seg * self.k, min(seg * self.k + self.k, self.steps)
+            # recompute u_{a+1..b} from the checkpoint (bit-identical to the forward)
+            L.call("tl_snapshot_load", self.CK + seg, L.FIELD["energy1"])
+            for n in range(a + 1, b + 1):
+                self._step()
+                L.call("tl_snapshot_save", self.SEG + (n - a - 1), L.FIELD["u"])
+            for n in range(b, a, -1):
+                # lambda_n = A^-1 g_n, from zero
+                L.call("tl_snapshot_load", self.RHS, L.FIELD["u0"])
+                L.call("tl_snapshot_load", self.ZERO, L.FIELD["u"])
+                self._solve()
+                L.call("tl_adjoint_face_grad", L.FIELD["u"], self.SEG + (n - a - 1),
+                       self.E0 if n == 1 else -1, self.GRAD)
+                L.call("tl_snapshot_save", self.RHS, L.FIELD["u"])   # g_{n-1} = lambda_n
+        out = np.zeros((self.ny + 2, self.nx + 2))
+        L.call("tl_snapshot_download", self.GRAD, out.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        return out[1:-1, 1:-1].copy()
+
+    # -- torch -------------------------------------------------------------------------
+    def apply(self, theta):
+        """u_N as a differentiable function of a torch tensor theta (ny, nx)."""
+        return _function()(theta, self)
+
+
+_FN = None
+
+
+def _function():
+    global _FN
+    if _FN is None:
+        import torch
+
+        class TeaLeafFunction(torch.autograd.Function):
+            """theta (log-density) -> u after N steps; backward by the adjoint."""
+
+            @staticmethod
+            def forward(ctx, theta, problem):
+                ctx.problem = problem
+                u = problem.forward(theta.detach().to("cpu", torch.float64).numpy())
+                return torch.from_numpy(u).to(device=theta.device, dtype=theta.dtype)
+
+            @staticmethod
+            def backward(ctx, grad_u):
+                g = ctx.problem.backward(grad_u.detach().to("cpu", torch.float64).numpy())
+                return torch.from_numpy(g).to(device=grad_u.device, dtype=grad_u.dtype), None
+
+        _FN = TeaLeafFunction.apply
+    return _FN

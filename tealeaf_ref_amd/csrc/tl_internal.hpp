@@ -76,6 +76,10 @@ struct State {
   long pitch32 = 0;
   size_t field32 = 0;          // floats per field
 
+  // snapshot slots (checkpoints and work arrays of the adjoint), each laid
+  // out exactly like a field allocation
+  std::vector<double *> snaps;
+
   // profiling
   bool prof = false;
   struct Ev { hipEvent_t a, b; int cls; };
@@ -126,6 +130,9 @@ void prof_end(int cls, hipEvent_t a);
 // finish a reduction: sum `n` partials of `slot` (deterministically) into
 // dscal[idx]; returns after copying it to the host (synchronises)
 int finish_sums(const int *slots, int nslots, int n, double *out);
+
+// field of a TL_FIELD_* id (the current buffer of ping-ponged fields)
+Field *field_by_id(int id);
 
 }  // namespace tl
 

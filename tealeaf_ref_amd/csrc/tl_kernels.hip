@@ -9,7 +9,6 @@
 #include "tl_stencil.hpp"
 
 namespace tl {
-Field *field_by_id(int id);
 dev::Geo geo(int cols_per_tile = BLOCK);
 
 dev::Geo geo(int cols_per_tile) {

@@ -170,6 +170,7 @@ extern "C" int tl_finalize(void) {
   if (s.hctl) (void)hipHostFree(s.hctl);
   if (s.hist) (void)hipFree(s.hist);
   if (s.f32) (void)hipFree(s.f32);
+  for (double *q : s.snaps) (void)hipFree(q);
   for (auto &ev : s.ev_pending) {
     (void)hipEventDestroy(ev.a);
     (void)hipEventDestroy(ev.b);

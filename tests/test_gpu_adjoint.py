@@ -1,0 +1,132 @@
+"""Adjoint of N implicit steps (tl_adjoint.hip, tealeaf_ref_amd.adjoint)
+against the CPU reference adjoint (adjoint_ref.py, itself pinned by finite
+differences), against central finite differences of the GPU forward, and
+for checkpoint invariance, the torch binding and the mixed-precision form."""
+import numpy as np
+import pytest
+
+import adjoint_ref as R
+from tealeaf_ref_amd import _lib as L
+from tealeaf_ref_amd.adjoint import TeaLeafSteps
+
+pytestmark = pytest.mark.gpu
+
+
+def case(seed, ny=40, nx=48):
+    rng = np.random.default_rng(seed)
+    base = np.where(rng.random((ny // 8, nx // 8)) < 0.3, 0.1, 100.0)
+    rho = np.kron(base, np.ones((8, 8)))
+    theta = np.log(rho) + 0.1 * rng.standard_normal((ny, nx))
+    e0 = rng.uniform(0.1, 25.0, (ny, nx))
+    target = rng.uniform(0.0, 2.0, (ny, nx))
+    win = (slice(8, 24), slice(10, 34))
+    return theta, e0, target, win
+
+
+def seed_grad(u, target, win):
+    g = np.zeros_like(u)
+    g[win] = 2.0 * (u[win] - target[win])
+    return g
+
+
+def objective(u, target, win):
+    return float(((u[win] - target[win]) ** 2).sum())
+
+
+@pytest.fixture
+def problems():
+    made = []
+
+    def make(*a, **k):
+        p = TeaLeafSteps(*a, **k)
+        made.append(p)
+        return p
+
+    yield make
+    for p in made:
+        p.close()
+
+
+@pytest.mark.parametrize("coef", [L.CONDUCTIVITY, L.RECIP_CONDUCTIVITY])
+def test_forward_and_gradient_match_reference(problems, coef):
+    theta, e0, target, win = case(1 + coef)
+    steps, rx = 4, 0.5 if coef == L.RECIP_CONDUCTIVITY else 20.0
+    p = problems(e0, rx, rx, steps, coefficient=coef, eps=1e-15, checkpoint_every=3)
+    print(f"adjoint vs reference coef={coef}", flush=True)
+    u = p.forward(theta)
+    recip = coef == L.RECIP_CONDUCTIVITY
+    u_ref = R.forward(theta, e0, rx, rx, steps, recip)[-1]
+    assert np.max(np.abs(u - u_ref)) <= 1e-11 * np.max(np.abs(u_ref))
+    g = p.backward(seed_grad(u, target, win))
+    g_ref = R.gradient(theta, e0, rx, rx, steps, recip, seed_grad(u_ref, target, win))
+    err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
+    print(f"  max rel gradient error vs reference {err:.2e}", flush=True)
+    assert err <= 1e-9
+
+
+def test_gradient_matches_finite_differences(problems):
+    """Central differences with h = 1e-5 in log-density on cells inside the
+    window of J.  The target is the forward of a log-normally perturbed
+    density (sigma 0.1), as in the benchmark's adjoint case.  The difference
+    quotient carries the solver's own error (|du| / (h |du/dtheta|), |du| up
+    to eps * cond(A) |u|), so cells that barely couple to the window, where
+    |du/dtheta| is small, are not a sharp test."""
+    theta, e0, _, win = case(7)
+    steps, rx, h = 3, 20.0, 1e-5
+    p = problems(e0, rx, rx, steps, coefficient=L.CONDUCTIVITY, eps=1e-15, checkpoint_every=2)
+    rng = np.random.default_rng(3)
+    target = p.forward(theta + 0.1 * rng.standard_normal(theta.shape))
+    u = p.forward(theta)
+    g = p.backward(seed_grad(u, target, win))
+    cells = [(int(rng.integers(9, 23)), int(rng.integers(11, 33))) for _ in range(8)]
+    worst = 0.0
+    for c in cells:
+        tp, tm = theta.copy(), theta.copy()
+        tp[c] += h
+        tm[c] -= h
+        fd = (objective(p.forward(tp), target, win) - objective(p.forward(tm), target, win)) / (2 * h)
+        rel = abs(fd - g[c]) / max(abs(g[c]), 1e-300)
+        worst = max(worst, rel)
+        assert rel <= 1e-5, (c, fd, g[c])
+    print(f"  FD check on {len(cells)} cells: max rel err {worst:.2e}", flush=True)
+
+
+def test_checkpointing_is_bit_identical(problems):
+    theta, e0, target, win = case(11)
+    grads = []
+    for k in (1, 2, 5):
+        p = problems(e0, 10.0, 10.0, 5, checkpoint_every=k)
+        u = p.forward(theta)
+        grads.append(p.backward(seed_grad(u, target, win)))
+        p.close()
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+def test_torch_function(problems):
+    import torch
+    theta, e0, target, win = case(13)
+    p = problems(e0, 10.0, 10.0, 3, checkpoint_every=2)
+    dev = "cuda" if torch.cuda.is_available() else "cpu"
+    th = torch.tensor(theta, device=dev, requires_grad=True)
+    tg = torch.tensor(target, device=dev)
+    u = p.apply(th)
+    J = ((u[win] - tg[win]) ** 2).sum()
+    J.backward()
+    u_np = p.forward(theta)
+    g_np = p.backward(seed_grad(u_np, target, win))
+    assert np.array_equal(th.grad.cpu().numpy(), g_np)
+    assert abs(J.item() - objective(u_np, target, win)) <= 1e-12 * abs(J.item())
+
+
+def test_mixed_precision_adjoint(problems):
+    theta, e0, target, win = case(17)
+    grads = {}
+    for prec in ("fp64", "mixed"):
+        p = problems(e0, 20.0, 20.0, 4, checkpoint_every=2, precision=prec,
+                     eps=1e-15 if prec == "fp64" else 1e-11)
+        u = p.forward(theta)
+        grads[prec] = p.backward(seed_grad(u, target, win))
+        p.close()
+    rel = np.linalg.norm(grads["mixed"] - grads["fp64"]) / np.linalg.norm(grads["fp64"])
+    print(f"  mixed vs FP64 gradient rel L2 {rel:.2e}", flush=True)
+    assert rel <= 1e-7
