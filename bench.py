@@ -163,6 +163,97 @@ def precision_comparison(cells, steps, warmup, inner_steps=10):
     return out
 
 
+def deck_fields(cells, coefficient="density"):
+    """Density, energy (interior) and rx, ry of the seeded deck, generated by
+    the native host (generate_chunk) on the GPU and downloaded."""
+    import numpy as np
+    from tealeaf_ref_amd import _lib
+    from tealeaf_ref_amd.deckgen import bench_deck
+    from tealeaf_ref_amd.host import TeaHost
+    deck = bench_deck(cells, solver="cg", eps=1e-15, max_iters=10000, end_step=1,
+                      coefficient=coefficient)
+    with tempfile.TemporaryDirectory() as wd:
+        path = os.path.join(wd, "tea.in")
+        with open(path, "w") as f:
+            f.write(deck)
+        h = TeaHost(path, os.path.join(wd, "tea.out"))
+        rho = _lib.download_field("density", cells, cells)[1:-1, 1:-1].copy()
+        e1 = _lib.download_field("energy1", cells, cells)[1:-1, 1:-1].copy()
+        h.close()
+    dx = 10.0 / cells
+    dt = 0.004
+    return rho, e1, dt / (dx * dx), dt / (dx * dx)
+
+
+def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed=5):
+    """The adjoint case: gradient of J = sum over a seeded 256x256 window of
+    (u_N - u_target)^2 with respect to per-cell log-density, u_target from a
+    forward on a log-normally perturbed density (sigma 0.1); timed forward and
+    backward (checkpoint every `every` steps); central differences with step
+    h on n_fd seeded cells inside the window."""
+    import ctypes as C
+    import numpy as np
+    from tealeaf_ref_amd import _lib
+    from tealeaf_ref_amd.adjoint import TeaLeafSteps
+    rho, e0, rx, ry = deck_fields(cells)
+    rng = np.random.default_rng(seed)
+    theta = np.log(rho)
+    theta_p = theta + 0.1 * rng.standard_normal(theta.shape)
+    wy, wx = (int(v) for v in rng.integers(0, cells - 256, 2))
+    win = (slice(wy, wy + 256), slice(wx, wx + 256))
+    precs = ["fp64", "mixed"] if precision == "both" else [precision]
+    out = {"workload": f"{cells}x{cells}, {steps} steps, seeded deck, per-cell density "
+                       f"rho*exp(0.1 xi), window 256x256 at ({wx},{wy}), checkpoint every {every}"}
+    grads = {}
+    for prec in precs:
+        p = TeaLeafSteps(e0, rx, ry, steps, coefficient=_lib.CONDUCTIVITY,
+                         eps=1e-15 if prec == "fp64" else mixed_eps, checkpoint_every=every,
+                         precision=prec)
+        target = p.forward(theta_p)
+        t0 = time.perf_counter()
+        u = p.forward(theta)
+        t_f = time.perf_counter() - t0
+        its_f = p.iterations
+        seed_g = np.zeros_like(u)
+        seed_g[win] = 2.0 * (u[win] - target[win])
+        J = float(((u[win] - target[win]) ** 2).sum())
+        t0 = time.perf_counter()
+        g = p.backward(seed_g)
+        t_b = time.perf_counter() - t0
+        its_b = p.iterations
+        nbytes = C.c_double()
+        _lib.call("tl_device_bytes", C.byref(nbytes))
+        rec = {"J": J, "forward_s_per_step": t_f / steps,
+               "forward_backward_s_per_step": (t_f + t_b) / steps,
+               "forward_iterations_per_step": its_f / steps,
+               "backward_iterations_per_step": its_b / steps,
+               "device_bytes": nbytes.value}
+        fd_rng = np.random.default_rng(seed + 1)
+        errs = []
+        for _ in range(n_fd):
+            c = (int(fd_rng.integers(wy, wy + 256)), int(fd_rng.integers(wx, wx + 256)))
+            tp, tm = theta.copy(), theta.copy()
+            tp[c] += h
+            tm[c] -= h
+            up, um = p.forward(tp), p.forward(tm)
+            fd = (float(((up[win] - target[win]) ** 2).sum())
+                  - float(((um[win] - target[win]) ** 2).sum())) / (2 * h)
+            errs.append(abs(fd - g[c]) / abs(g[c]))
+            log(f"[bench] adjoint {prec} FD cell {c}: fd {fd:.10e} adjoint {g[c]:.10e} "
+                f"rel {errs[-1]:.2e}")
+        if errs:
+            rec.update({"fd_cells": n_fd, "fd_h": h, "fd_max_rel_err": max(errs),
+                        "fd_median_rel_err": float(np.median(errs))})
+        log(f"[bench] adjoint {cells}^2 {prec}: {rec}")
+        out[prec] = rec
+        grads[prec] = g
+        p.close()
+    if len(grads) == 2:
+        out["mixed_vs_fp64_gradient_rel_l2"] = float(
+            np.linalg.norm(grads["mixed"] - grads["fp64"]) / np.linalg.norm(grads["fp64"]))
+    return out
+
+
 def cpu_baseline(cells, solver, gpu_iters_per_step, cap):
     """Reference build (oracle/_ref) on the host cores: one timestep capped at
     `cap` iterations, profiler_on -> Time Per It; falls back to the C oracle
@@ -237,7 +328,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=4096)
     ap.add_argument("--solver", default="cg", choices=["cg"])
-    ap.add_argument("--cpu-cap", type=int, default=100, help="iterations of the CPU sample")
+    ap.add_argument("--cpu-cap", type=int, default=1500, help="iterations of the CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-sizes", default="4096",
                     help="comma-separated mesh sizes of the FP64 vs FP32-refined comparison "
@@ -245,6 +336,15 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="counter-profiling mode: one CG step capped at this many iterations "
                          "plus copy-kernel calibration launches; prints a short JSON line")
+    ap.add_argument("--adjoint-cells", type=int, default=2048,
+                    help="mesh of the adjoint case (0 to skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=20)
+    ap.add_argument("--adjoint-every", type=int, default=5, help="checkpoint interval")
+    ap.add_argument("--adjoint-fd", type=int, default=4, help="finite-difference cells")
+    ap.add_argument("--adjoint-h", type=float, default=1e-6)
+    ap.add_argument("--adjoint-precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
+                    help="true relative residual of the mixed-precision solves")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-warmup", type=int, default=1)
     args = ap.parse_args()
@@ -300,6 +400,10 @@ def main():
     if sizes:
         out["precision_comparison"] = [
             precision_comparison(n, args.mixed_steps, args.mixed_warmup) for n in sizes]
+    if args.adjoint_cells:
+        out["adjoint"] = adjoint_bench(args.adjoint_cells, args.adjoint_steps, args.adjoint_every,
+                                       args.adjoint_fd, args.adjoint_h, args.adjoint_precision,
+                                       args.adjoint_mixed_eps)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap)

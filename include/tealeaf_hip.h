@@ -271,6 +271,10 @@ int tl_snapshot_download(int slot, double *host, int host_halo);
  * lambda is read from a field, u_n, e_0 and grad from slots. */
 int tl_adjoint_face_grad(int lambda_field, int u_slot, int e0_slot, int grad_slot);
 
+/* Device memory the library holds (fields, FP32 fields, snapshot slots,
+ * reduction buffers), in bytes.  No counterpart in the reference. */
+int tl_device_bytes(double *bytes);
+
 /* ---- timing of the dominant kernels (HIP events on the library stream) --- */
 /* Enables per-launch HIP-event timing of the solver kernels. */
 int tl_profile_enable(int on);

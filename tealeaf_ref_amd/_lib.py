@@ -80,7 +80,7 @@ def lib():
             "tl_snapshot_reserve": [I], "tl_snapshot_save": [I, I], "tl_snapshot_load": [I, I],
             "tl_snapshot_fill": [I, D], "tl_snapshot_upload": [I, PD, I],
             "tl_snapshot_download": [I, PD, I], "tl_adjoint_face_grad": [I, I, I, I],
-            "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
+            "tl_device_bytes": [PD], "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
             "tl_profile_reset": [],
         }
         for name, args in sig.items():
@@ -105,6 +105,14 @@ def call(name, *args):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def download_field(name, nx, ny, halo=1):
+    """Copy a field of the chunk the library currently holds (whoever set it
+    up) to a new host array (ny+2*halo, nx+2*halo)."""
+    a = np.zeros((ny + 2 * halo, nx + 2 * halo))
+    call("tl_download", FIELD[name], _dptr(a), halo)
+    return a
 
 
 class Device:

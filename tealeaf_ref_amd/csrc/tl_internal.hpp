@@ -75,6 +75,7 @@ struct State {
   float *f32 = nullptr;
   long pitch32 = 0;
   size_t field32 = 0;          // floats per field
+  size_t f32_bytes = 0;        // the whole FP32 allocation
 
   // snapshot slots (checkpoints and work arrays of the adjoint), each laid
   // out exactly like a field allocation

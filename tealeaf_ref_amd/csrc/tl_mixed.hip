@@ -49,6 +49,7 @@ int ensure_f32() {
   s.field32 = (size_t)(s.pitch32 * (s.ny + 2) + 64);
   const size_t bytes = sizeof(float) * s.field32 * NF32;
   TL_HIP(hipMalloc((void **)&s.f32, bytes));
+  s.f32_bytes = bytes;
   // columns and rows outside the interior stay zero for good
   TL_HIP(hipMemsetAsync(s.f32, 0, bytes, s.stream));
   return TL_OK;

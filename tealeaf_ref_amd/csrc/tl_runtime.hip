@@ -314,6 +314,20 @@ extern "C" int tl_profile_enable(int on) {
   return TL_OK;
 }
 
+extern "C" int tl_device_bytes(double *bytes) {
+  TL_REQUIRE_INIT();
+  if (!bytes) return fail(TL_ERR_ARG, "null pointer");
+  const State &s = S();
+  double b = 0.0;
+  for (int i = 0; all_fields(i); ++i) b += (double)s.field_bytes;
+  b += (double)s.snaps.size() * (double)s.field_bytes;
+  b += (double)s.f32_bytes;
+  b += sizeof(double) * (double)MAX_GRID * NPART;
+  if (s.hist) b += sizeof(double) * 3.0 * (double)s.hist_len;
+  *bytes = b;
+  return TL_OK;
+}
+
 extern "C" int tl_profile_get(int cls, long *launches, double *total_ms) {
   TL_REQUIRE_INIT();
   if (cls < 0 || cls >= 8) return fail(TL_ERR_ARG, "bad kernel class");
