@@ -34,6 +34,20 @@ CG_UR_BYTES_PER_CELL = 48.0     # calc_ur: u, p, r, w in; u, r out
 CG_ITER_BYTES_PER_CELL = 96.0
 
 
+def measured_traffic(kernel, cells):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated), or None."""
+    if cells != 4096:
+        return None
+    path = os.path.join(ROOT, "profiles", "cg_fp64_4096_hbm_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch %s: (\d+)" % re.escape(kernel), text)
+    return float(m.group(1)) if m else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -390,7 +404,9 @@ def main():
         "cell_iterations_per_s": cells * exec_iters / elapsed,
         "roofline": {"bound": "hbm", "kernel": "cg calc_w (w = A p, p = r + beta p, p.w)",
                      "achieved": w_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": (w_gbs / PEAK_HBM_GBS) if w_gbs else None, "traffic": None,
+                     "frac": (w_gbs / PEAK_HBM_GBS) if w_gbs else None,
+                     "traffic": measured_traffic("k_loop_w", args.cells),
+                     "traffic_source": "profiles/cg_fp64_4096_hbm_traffic.md",
                      "bytes_per_cell": CG_W_BYTES_PER_CELL,
                      "avg_launch_ms": ms_w / n_w if n_w else None, "launches": n_w,
                      "calc_ur_achieved": ur_gbs,
