@@ -52,13 +52,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_run(cells, solver, steps, warmup, workdir):
+def gpu_run(cells, solver, steps, warmup, workdir, **deck_kw):
     from tealeaf_ref_amd import _lib
     from tealeaf_ref_amd.deckgen import bench_deck
     from tealeaf_ref_amd.host import TeaHost
 
     deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=10000,
-                      end_step=steps + warmup, summary_frequency=1)
+                      end_step=steps + warmup, summary_frequency=1, **deck_kw)
     deck_path = os.path.join(workdir, "tea.in")
     with open(deck_path, "w") as f:
         f.write(deck)
@@ -72,17 +72,19 @@ def gpu_run(cells, solver, steps, warmup, workdir):
     _lib.call("tl_profile_reset")
     _lib.call("tl_profile_enable", 1)
     iters = []
+    rows = []
     t0 = time.perf_counter()
     for i in range(steps):
         h.step()
-        it = h.last()["iterations"]
+        rows.append(h.last())
+        it = rows[-1]["iterations"]
         iters.append(it)
         log(f"[bench] step {i + 1}: {it:.0f} iterations")
     _lib.call("tl_synchronize")
     elapsed = time.perf_counter() - t0
     import ctypes as C
-    prof = {}
-    for cls in (0, 1):
+    prof = {"rows": rows}
+    for cls in range(8):
         n = C.c_long()
         ms = C.c_double()
         _lib.call("tl_profile_get", cls, C.byref(n), C.byref(ms))
@@ -90,6 +92,36 @@ def gpu_run(cells, solver, steps, warmup, workdir):
     _lib.call("tl_profile_enable", 0)
     h.close()
     return elapsed, iters, prof
+
+
+# algorithmic bytes per cell of the kernel classes of tl_profile_get
+CLASS_BYTES = {0: ("cg calc_w", 48.0), 1: ("cg calc_ur", 48.0), 2: ("ppcg inner sweep", 64.0),
+               3: ("chebyshev iterate", 56.0), 4: ("jacobi", 40.0), 5: ("ppcg calc_w", 48.0),
+               6: ("ppcg calc_ur + store_r", 56.0)}
+
+
+def solver_runs(cells, steps, warmup):
+    """Chebyshev (20 CG presteps) and PPCG (10 inner steps) on the same
+    4096^2 deck at tl_eps 1e-15, tl_max_iters 10000: seconds per step,
+    iterations and the per-kernel-class HBM rates (HIP events)."""
+    out = {}
+    for solver, kw in (("chebyshev", {"presteps": 20}), ("ppcg", {"inner_steps": 10})):
+        with tempfile.TemporaryDirectory() as wd:
+            elapsed, iters, prof = gpu_run(cells, solver, steps, warmup, wd, **kw)
+        rec = {"s_per_step": elapsed / steps, "iterations_per_step": sum(iters) / steps}
+        if solver == "ppcg":
+            rec["sweeps_per_step"] = sum(r["ppcg_inner"] for r in prof["rows"]) / steps
+        kern = {}
+        for cls, (name, bpc) in CLASS_BYTES.items():
+            n, ms = prof[cls]
+            if n:
+                kern[name] = {"launches": n, "avg_launch_ms": ms / n,
+                              "achieved_gbs": bpc * cells * cells * n / (ms * 1e-3) / 1e9,
+                              "bytes_per_cell": bpc}
+        rec["kernels"] = kern
+        log(f"[bench] {cells}^2 {solver}: {rec}")
+        out[solver] = rec
+    return out
 
 
 def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
@@ -359,6 +391,8 @@ def main():
     ap.add_argument("--adjoint-precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
                     help="true relative residual of the mixed-precision solves")
+    ap.add_argument("--solver-steps", type=int, default=1,
+                    help="timed steps of the Chebyshev and PPCG runs (0 to skip)")
     ap.add_argument("--mixed-steps", type=int, default=2)
     ap.add_argument("--mixed-warmup", type=int, default=1)
     args = ap.parse_args()
@@ -412,6 +446,8 @@ def main():
                      "calc_ur_achieved": ur_gbs,
                      "calc_ur_avg_launch_ms": ms_ur / n_ur if n_ur else None},
     }
+    if args.solver_steps:
+        out["solvers"] = solver_runs(args.cells, args.solver_steps, 1)
     sizes = [int(x) for x in args.mixed_sizes.split(",") if x.strip()]
     if sizes:
         out["precision_comparison"] = [

@@ -60,6 +60,7 @@ struct State {
       Kx, Ky, volume, cp, bfp;
   Field u[2], p[2], sd[2];
   int ucur = 0, pcur = 0, sdcur = 0;
+  int walk_dir = 1;        // row direction of the next Chebyshev walk
 
   // reductions: partial sums per block, and small device scalars
   double *partials = nullptr;  // NPART arrays of MAX_GRID doubles
@@ -134,6 +135,11 @@ int finish_sums(const int *slots, int nslots, int n, double *out);
 
 // field of a TL_FIELD_* id (the current buffer of ping-ponged fields)
 Field *field_by_id(int id);
+
+// Chebyshev iterations on the vector walk (tl_loops.hip): 1 = done,
+// 0 = not applicable here, < 0 = -(error code)
+int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_first, int count,
+                    int want_norm, double *norm);
 
 }  // namespace tl
 

@@ -839,6 +839,9 @@ extern "C" int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int
   if (!ch_alphas || !ch_betas || step_first < 1 || count < 0)
     return fail(TL_ERR_ARG, "tl_cheby_run: bad arguments");
   if (want_norm && !norm) return fail(TL_ERR_ARG, "tl_cheby_run: null norm");
+  const int fused = cheby_run_fused(ch_alphas, ch_betas, step_first, count, want_norm, norm);
+  if (fused < 0) return -fused;
+  if (fused) return TL_OK;
   for (int i = 0; i < count; ++i) {
     const int st = step_first + i - 1;
     const int last = (i == count - 1) && want_norm;

@@ -115,15 +115,15 @@ constexpr bool diff_form() {
 #define TL_PF_SWEEP 0
 #endif
 // Stores: written fields are re-read by the next kernel, so plain stores that
-// leave them in the 256 MB Infinity Cache beat streaming stores, except for
-// the FP64 PPCG sweeps, which write three fields per pass (measured at
-// 4096^2).  The CG update walks its rows in reverse so that it reads first
-// what the w = A p pass wrote last.
+// leave them in the 256 MB Infinity Cache beat streaming stores (measured at
+// 4096^2).  Consecutive passes walk their rows in opposite directions (the
+// CG and PPCG updates in reverse, the PPCG sweeps alternately), so that each
+// pass reads first what the one before wrote last.
 #ifndef TL_ST_NT_LOOP
 #define TL_ST_NT_LOOP 0
 #endif
 #ifndef TL_ST_NT_SWEEP
-#define TL_ST_NT_SWEEP 1
+#define TL_ST_NT_SWEEP 0
 #endif
 #ifndef TL_UR_REV
 #define TL_UR_REV 1
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int t
                                                    const double *__restrict__ prev_part,
                                                    int nprev, double *__restrict__ out_part,
                                                    Ctl *ctl, double *__restrict__ hrro,
-                                                   double *__restrict__ hbeta) {
+                                                   double *__restrict__ hbeta, int dir) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   if (!first && ctl->stop) return;
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int t
         for (int e = 0; e < V; ++e)
           if (tt.c + e < G.nx) acc += (double)wv[e] * (double)v[e];
       }
-    });
+    }, dir);
   }
   if (h.stop) return;
   acc = dev::block_sum(acc, lds);
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
     T *__restrict__ rtemp, T *__restrict__ utemp, const T *__restrict__ r,
     const T *__restrict__ r_store, T *__restrict__ z, const T *__restrict__ Mi,
     const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry,
-    double *__restrict__ part_rrn, double *__restrict__ part_zr, const Ctl *ctl) {
+    double *__restrict__ part_rrn, double *__restrict__ part_zr, const Ctl *ctl, int dir) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   if (ctl && ctl->stop) return;  // no control block: sweeps outside a loop
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
             azr += (double)ut[e] * (double)rv[e];
           }
       }
-    });
+    }, dir);
   }
   if (last_sweep) {
     arrn = dev::block_sum(arrn, lds);
@@ -456,7 +456,7 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
     prof_begin(0, &ev);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
                        st->rro, b.cg_src, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
-                       part(P2), nb, part(P1), ctl, h_rro(), h_beta());
+                       part(P2), nb, part(P1), ctl, h_rro(), h_beta(), 1);
     prof_end(0, ev);
     prof_begin(1, &ev);
     hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
@@ -469,7 +469,7 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
     const int cin = c0 ^ ((n - st->n_first) & 1);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, st->rro, b.cg_src,
                        b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P2), nb,
-                       part(P1), ctl, h_rro(), h_beta());
+                       part(P1), ctl, h_rro(), h_beta(), 1);
     TL_HIP(hipGetLastError());
     return TL_OK;
   };
@@ -522,7 +522,7 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
     prof_begin(5, &ev);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
                        rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3),
-                       nb, part(P1), ctl, h_rro(), h_beta());
+                       nb, part(P1), ctl, h_rro(), h_beta(), 1);
     prof_end(5, ev);
     prof_begin(6, &ev);
     hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, rro_first,
@@ -534,7 +534,8 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
       hipLaunchKernelGGL(ksw, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
                          i == inner ? 1 : 0, 1, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
                          b.diag, src_sd, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r, b.r_store,
-                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2), ctl);
+                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2), ctl,
+                         (i & 1) ? 1 : -1);
       prof_end(2, ev);
       sd = 1 - sd;
     }
@@ -545,7 +546,7 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
     const int cin = c0 ^ ((n - st->n_first) & 1);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, rro_first, b.z,
                        b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3), nb,
-                       part(P1), ctl, h_rro(), h_beta());
+                       part(P1), ctl, h_rro(), h_beta(), 1);
     TL_HIP(hipGetLastError());
     return TL_OK;
   };
@@ -581,7 +582,7 @@ int run_ppcg_precondition(const LoopBufs<T> &b, const double *alphas, const doub
                        i == inner ? 1 : 0, 0, theta_r, (T)alphas[i - 1], (T)betas[i - 1], b.diag,
                        b.diag ? b.z : b.r, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r,
                        b.r_store, b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2),
-                       (const Ctl *)nullptr);
+                       (const Ctl *)nullptr, (i & 1) ? 1 : -1);
     sd = 1 - sd;
   }
   *b.sdcur = sd;

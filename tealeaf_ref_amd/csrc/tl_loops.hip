@@ -86,6 +86,128 @@ __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev
 #define TL_V64 2
 #endif
 
+// ---------------------------------------------------------------------------
+// Chebyshev iteration (kernels/tea_leaf_cheby_kernel.f90:111-200) on the
+// vector walk: w = A u, r = u0 - w, (z = Mi r), p = alpha p + beta (r|z),
+// u' = u + p into the other u buffer.  w, r (and z) are stored, and the
+// partials of r.r written, only on the last iteration of a run: nothing reads
+// them in between.  The halo ring of u is carried over to the new buffer, as
+// the per-routine kernel does.  For chunks whose external faces are zero
+// (not reflective), where the halo values never enter the operator.
+// ---------------------------------------------------------------------------
+#ifndef TL_PF_CHEBY
+#define TL_PF_CHEBY 0
+#endif
+
+template <bool NT>
+__global__ __launch_bounds__(BLOCK) void k_cheby_v(Geo G, const double *__restrict__ uin,
+                                                    double *__restrict__ uout,
+                                                    const double *__restrict__ u0,
+                                                    const double *__restrict__ Kx,
+                                                    const double *__restrict__ Ky,
+                                                    double *__restrict__ p, double *__restrict__ w,
+                                                    double *__restrict__ r, double *__restrict__ z,
+                                                    const double *__restrict__ Mi, double rx,
+                                                    double ry, double alpha, double beta, int diag,
+                                                    int last, double *__restrict__ out_part,
+                                                    int dir) {
+  typedef typename dev::Vec<double, 2>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<2> tt = dev::tilev_of<2>(G, t);
+    const int cc = tt.c <= G.nx ? tt.c : (G.nx / 2) * 2;
+    struct Side {
+      VT u0, p, mi;
+    };
+    auto side = [&](int row) -> Side {
+      const long i = (long)row * P + cc;
+      Side e;
+      e.u0 = dev::ldv<NT, double, 2>(u0 + i);
+      e.p = dev::ldv<NT, double, 2>(p + i);
+      e.mi = diag ? dev::ldv<NT, double, 2>(Mi + i) : VT(0);
+      return e;
+    };
+    auto ldop = [&](int col, int row) -> VT {
+      return dev::ldv<NT, double, 2>(uin + (long)row * P + col);
+    };
+    auto ld1 = [&](int col, int row) -> double { return uin[(long)row * P + col]; };
+    dev::walkv<NT, double, 2, false, TL_PF_CHEBY, true>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, side, [&](int y, VT v, VT wv, const Side &e) {
+      if (tt.c >= G.nx) return;
+      const long row = (long)y * P;
+      const VT rv = e.u0 - wv;
+      const VT src = diag ? e.mi * rv : rv;
+      const VT pv = alpha * e.p + beta * src;
+      dev::stv<false, double, 2>(p + row, tt.c, G.nx, pv);
+      dev::stv<false, double, 2>(uout + row, tt.c, G.nx, v + pv);
+      if (last) {
+        dev::stv<false, double, 2>(w + row, tt.c, G.nx, wv);
+        dev::stv<false, double, 2>(r + row, tt.c, G.nx, rv);
+        if (diag) dev::stv<false, double, 2>(z + row, tt.c, G.nx, src);
+        acc += rv.x * rv.x;
+        if (tt.c + 1 < G.nx) acc += rv.y * rv.y;
+      }
+      // carry the halo ring of u to the new buffer
+      if (tt.c == 0) uout[row - 1] = uin[row - 1];
+      if (tt.c + 2 >= G.nx) uout[row + G.nx] = uin[row + G.nx];
+      for (int side_row = 0; side_row < 2; ++side_row) {
+        if (y != (side_row ? G.ny - 1 : 0)) continue;
+        const long hr = side_row ? (long)G.ny * P : -P;
+        uout[hr + tt.c] = uin[hr + tt.c];
+        if (tt.c + 1 < G.nx) uout[hr + tt.c + 1] = uin[hr + tt.c + 1];
+      }
+    }, dir);
+  }
+  if (last) {
+    acc = dev::block_sum(acc, lds);
+    if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+  }
+}
+
+}  // namespace
+
+namespace tl {
+// Chebyshev iterations through the vector walk; returns 1 if it ran them,
+// 0 if the configuration needs the per-routine kernel (reflective faces,
+// block preconditioner, internal faces), or a negative error.
+int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_first, int count,
+                    int want_norm, double *norm) {
+  State &s = S();
+  if (s.reflective || !s.zb_all || s.precond == TL_PREC_JAC_BLOCK) return 0;
+  const Geo G = geo(2 * BLOCK);
+  const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
+  const int diag = s.precond == TL_PREC_JAC_DIAG;
+  const bool nt = use_nontemporal();
+  auto k = nt ? k_cheby_v<true> : k_cheby_v<false>;
+  for (int i = 0; i < count; ++i) {
+    const int st = step_first + i - 1;
+    const int last = i == count - 1;
+    const int in = s.ucur, outb = 1 - s.ucur;
+    hipEvent_t ev;
+    prof_begin(3, &ev);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
+                       s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
+                       s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], diag, last, part(P0),
+                       s.walk_dir);
+    s.walk_dir = -s.walk_dir;  // the next iteration starts on the rows written last
+    prof_end(3, ev);
+    if (hipGetLastError() != hipSuccess) return -fail(TL_ERR_HIP, "k_cheby_v launch");
+    s.ucur = outb;
+  }
+  if (want_norm && count > 0) {
+    const int slot = P0;
+    const int rc = finish_sums(&slot, 1, nb, norm);
+    if (rc) return -rc;
+  }
+  return 1;
+}
+}  // namespace tl
+
+namespace {
+
 // The FP64 fields as one buffer set of the shared loop kernels.
 LoopBufs<double> fp64_bufs() {
   State &s = S();

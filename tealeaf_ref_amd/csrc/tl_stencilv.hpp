@@ -96,6 +96,8 @@ struct NoExtra {};
 // operator.  With PF every load of row y+1's step (operand row y+2, faces,
 // edge cells, extra) is issued before row y is computed, so one row of loads
 // is always in flight behind the arithmetic; the body itself must not load.
+// dir = -1 walks the rows from the top of the tile down (a pass that follows
+// one which ended at the top then reads the freshest rows first).
 // Lanes wholly outside [0, nx) run the loop too (their values feed the
 // shuffles) but must not store.
 // With LDSX the cells just outside a wave that belong to another wave of the
@@ -106,7 +108,7 @@ template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, typ
           typename LD1, typename EXTRA, typename BODY>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
-                             EXTRA extra, BODY body) {
+                             EXTRA extra, BODY body, int dir = 1) {
   typedef typename Vec<T, V>::type VT;
   typedef decltype(extra(0)) E;
   constexpr int NW = BLOCK / WAVE;
@@ -114,8 +116,10 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
   const long P = g.pitch;
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
   const int c = tt.c, nx = g.nx, ny = g.ny;
-  const int y0 = tt.y0, y1 = tt.y1;
-  if (y0 >= y1) return;
+  const int count = tt.y1 - tt.y0;
+  if (count <= 0) return;
+  const bool fwd = dir > 0;
+  const int ys = fwd ? tt.y0 : tt.y1 - 1;  // first row walked
   // lanes past the mesh clamp their column so addresses stay in the row
   const int cl = c <= nx ? c : (nx / V) * V;
   const int xr = tt.x0 + WAVE * V;  // first column right of this wave
@@ -123,28 +127,33 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
   const bool need_left = lane == 0 && tt.x0 > 0 && edge_l;
   const bool need_right = lane == WAVE - 1 && xr < nx && edge_r;
   const bool need_kxr = lane == WAVE - 1 && xr <= nx && edge_r;
-  // state of row y: operand rows y-1, y, y+1 (mirrored at the mesh edge),
-  // faces Ky(y), Ky(y+1), Kx(y), the cells left and right of the wave,
-  // Kx right of the wave, and the extra data
-  VT vd = ldv_op(cl, y0 > 0 ? y0 - 1 : y0);
-  VT vc = ldv_op(cl, y0);
-  VT vu = y0 + 1 < ny ? ldv_op(cl, y0 + 1) : vc;
-  VT kyd = ldv<NT, T, V>(Ky + (long)y0 * P + cl);
-  VT kyu = ldv<NT, T, V>(Ky + (long)(y0 + 1) * P + cl);
-  VT kx = ldv<NT, T, V>(Kx + (long)y0 * P + cl);
-  T el = need_left ? ld1_op(tt.x0 - 1, y0) : T(0);
-  T er = need_right ? ld1_op(xr, y0) : T(0);
-  T ek = need_kxr ? Kx[(long)y0 * P + xr] : T(0);
-  E ex = extra(y0);
-  for (int y = y0; y < y1; ++y) {
-    const int yn = y + 1;
-    const bool more = yn < y1;
-    VT vu2 = vu, kyu2 = kyu, kx2 = kx;
+  // Ky face between row y and the next row in walking direction
+  auto face_ahead = [&](int y) -> long { return (long)(fwd ? y + 1 : y) * P + cl; };
+  // state of row y: operand rows behind (y - dir), y and ahead (y + dir),
+  // mirrored at the mesh edge; the Ky faces on both sides; Kx(y); the cells
+  // left and right of the wave; Kx right of the wave; the extra data
+  VT vc = ldv_op(cl, ys);
+  const int yb = ys - dir, ya = ys + dir;
+  VT vb = (yb >= 0 && yb < ny) ? ldv_op(cl, yb) : vc;
+  VT va = (ya >= 0 && ya < ny) ? ldv_op(cl, ya) : vc;
+  VT kb = ldv<NT, T, V>(Ky + (long)(fwd ? ys : ys + 1) * P + cl);
+  VT ka = ldv<NT, T, V>(Ky + face_ahead(ys));
+  VT kx = ldv<NT, T, V>(Kx + (long)ys * P + cl);
+  T el = need_left ? ld1_op(tt.x0 - 1, ys) : T(0);
+  T er = need_right ? ld1_op(xr, ys) : T(0);
+  T ek = need_kxr ? Kx[(long)ys * P + xr] : T(0);
+  E ex = extra(ys);
+  for (int k = 0; k < count; ++k) {
+    const int y = ys + k * dir;
+    const int yn = y + dir;
+    const bool more = k + 1 < count;
+    VT va2 = va, ka2 = ka, kx2 = kx;
     T el2 = el, er2 = er, ek2 = ek;
     E ex2 = ex;
     auto load_next = [&]() {
-      vu2 = yn + 1 < ny ? ldv_op(cl, yn + 1) : vu;
-      kyu2 = ldv<NT, T, V>(Ky + (long)(yn + 1) * P + cl);
+      const int yna = yn + dir;
+      va2 = (yna >= 0 && yna < ny) ? ldv_op(cl, yna) : va;
+      ka2 = ldv<NT, T, V>(Ky + face_ahead(yn));
       kx2 = ldv<NT, T, V>(Kx + (long)yn * P + cl);
       if (need_left) el2 = ld1_op(tt.x0 - 1, yn);
       if (need_right) er2 = ld1_op(xr, yn);
@@ -156,7 +165,7 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     T right = __shfl_down(vc[0], 1, WAVE);
     T kxr = __shfl_down(kx[0], 1, WAVE);
     if (LDSX) {
-      const int par = y & 1;
+      const int par = k & 1;
       if (lane == 0) {
         xch[par][wave][0] = vc[0];
         xch[par][wave][2] = kx[0];
@@ -173,6 +182,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     if (need_right) right = er;
     if (need_kxr) kxr = ek;
     if (c == 0) left = vc[0];  // mirror at the left edge
+    const VT &vd = fwd ? vb : va, &vu = fwd ? va : vb;
+    const VT &kyd = fwd ? kb : ka, &kyu = fwd ? ka : kb;
     VT w;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -188,11 +199,11 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     }
     body(y, vc, w, ex);
     if (!PF && more) load_next();
-    vd = vc;
-    vc = vu;
-    vu = vu2;
-    kyd = kyu;
-    kyu = kyu2;
+    vb = vc;
+    vc = va;
+    va = va2;
+    kb = ka;
+    ka = ka2;
     kx = kx2;
     el = el2;
     er = er2;
