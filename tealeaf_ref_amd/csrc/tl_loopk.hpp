@@ -16,6 +16,7 @@
 #include "tl_internal.hpp"
 #include "tl_stencil.hpp"
 #include "tl_stencilv.hpp"
+#include "tl_sweep2.hpp"
 
 namespace tl {
 
@@ -411,6 +412,78 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
   }
 }
 
+// Inner sweeps i and i+1 in one pass (tl_sweep2.hpp): reads sdin, rtin
+// (r on the first sweep), utemp; writes sdout, rtout, utemp.  first/last as
+// in k_pp_sweep, applying to sweep i / sweep i+1.
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_pp_sweep2(
+    Geo G, int first_sweep, int last_sweep, int want_rrn, T theta_r, T a1, T b1, T a2, T b2,
+    int diag, const T *__restrict__ src, const T *__restrict__ sdin, T *__restrict__ sdout,
+    const T *__restrict__ rtin, T *__restrict__ rtout, T *__restrict__ utemp,
+    const T *__restrict__ r, const T *__restrict__ r_store, T *__restrict__ z,
+    const T *__restrict__ Mi, const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry,
+    double *__restrict__ part_rrn, double *__restrict__ part_zr, const Ctl *ctl, int dir) {
+  typedef typename dev::Vec<T, V>::type VT;
+  constexpr bool ST = TL_ST_NT_SWEEP && sizeof(T) == 8;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl && ctl->stop) return;
+  const long P = G.pitch;
+  double arrn = 0.0, azr = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    auto opA = [&](int col, int row) -> VT {
+      const long i = (long)row * P + col;
+      return first_sweep ? dev::ldv<NT, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
+    };
+    auto opA1 = [&](int col, int row) -> T {
+      const long i = (long)row * P + col;
+      return first_sweep ? src[i] * theta_r : sdin[i];
+    };
+    auto rtA = [&](int col, int row) -> VT {
+      const long i = (long)row * P + col;
+      return first_sweep ? dev::ldv<NT, T, V>(r + i) : dev::ldv<NT, T, V>(rtin + i);
+    };
+    auto rtA1 = [&](int col, int row) -> T {
+      const long i = (long)row * P + col;
+      return first_sweep ? r[i] : rtin[i];
+    };
+    dev::walk_pair<NT, T, V, diff_form<T>()>(
+        G, tt, Kx, Ky, Mi, diag, rx, ry, a1, b1, opA, opA1, rtA, rtA1,
+        [&](int y, const VT &s1, const VT &r1, const VT &s0, const VT &w, const VT &mi) {
+          if (tt.c >= G.nx) return;
+          const long row = (long)y * P;
+          const VT r2 = r1 - w;
+          const VT s2 = diag ? a2 * s1 + (b2 * mi) * r2 : a2 * s1 + b2 * r2;
+          const VT ut0 = first_sweep ? s0 : dev::ldv<NT, T, V>(utemp + row + tt.c);
+          const VT ut = (ut0 + s1) + s2;
+          dev::stv<(NT && ST), T, V>(rtout + row, tt.c, G.nx, r2);
+          dev::stv<(NT && ST), T, V>(sdout + row, tt.c, G.nx, s2);
+          dev::stv<(NT && ST), T, V>(utemp + row, tt.c, G.nx, ut);
+          if (last_sweep) {
+            dev::stv<(NT && ST), T, V>(z + row, tt.c, G.nx, ut);
+            const VT rv = dev::ldv<NT, T, V>(r + row + tt.c);
+            const VT rs = want_rrn ? dev::ldv<NT, T, V>(r_store + row + tt.c) : VT(0);
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (tt.c + e < G.nx) {
+                if (want_rrn) arrn += ((double)rv[e] - (double)rs[e]) * (double)ut[e];
+                azr += (double)ut[e] * (double)rv[e];
+              }
+          }
+        },
+        dir);
+  }
+  if (last_sweep) {
+    arrn = dev::block_sum(arrn, lds);
+    azr = dev::block_sum(azr, lds);
+    if (threadIdx.x == 0) {
+      part_rrn[blockIdx.x] = arrn;
+      part_zr[blockIdx.x] = azr;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host loops over one set of buffers (FP64 fields or the FP32 inner fields)
 // ---------------------------------------------------------------------------
@@ -428,6 +501,61 @@ struct LoopBufs {
 };
 
 inline int abs_blocks(const Geo &G) { return G.blocks < 0 ? -G.blocks : G.blocks; }
+
+// Two inner sweeps per pass unless TL_SWEEP_PAIRS=0 (read on every call).
+inline bool sweep_pairs() {
+  const char *e = getenv("TL_SWEEP_PAIRS");
+  return !e || atoi(e) != 0;
+}
+
+// Queue the `inner` sweeps of one PPCG outer iteration (or of a preconditioner
+// application when ctl is null): pairs of sweeps in one pass each, a single
+// pass for an odd last one.  sd ping-pongs per pass; rtemp alternates
+// between b.rtemp and b.w (w is dead between the update and the next w pass).
+// Passes walk in alternating directions starting with `dir`; without pairs,
+// both sweeps of what would be a pair take that pair's direction, so the two
+// modes produce the same bits.
+template <typename T, int V>
+int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas, int inner,
+                 T theta_r, int want_rrn, const Ctl *ctl, int &sd, T *&rt_cur, int dir) {
+  State &s = S();
+  const Geo G = b.G;
+  const int nb = abs_blocks(G);
+  const bool nt = use_nontemporal();
+  auto k1 = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
+  auto k2 = nt ? k_pp_sweep2<true, T, V> : k_pp_sweep2<false, T, V>;
+  const T *src = b.diag ? b.z : b.r;
+  const bool pairs = sweep_pairs();
+  for (int i = 1; i <= inner;) {
+    hipEvent_t ev;
+    prof_begin(2, &ev);
+    if (pairs && i + 1 <= inner) {
+      T *rt_next = rt_cur == b.rtemp ? b.w : b.rtemp;
+      hipLaunchKernelGGL(k2, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
+                         i + 1 == inner ? 1 : 0, want_rrn, theta_r, (T)alphas[i - 1],
+                         (T)betas[i - 1], (T)alphas[i], (T)betas[i], b.diag, src, b.sd[sd],
+                         b.sd[1 - sd], rt_cur, rt_next, b.utemp, b.r, b.r_store, b.z, b.Mi, b.Kx,
+                         b.Ky, b.rx, b.ry, part(P3), part(P2), ctl, dir);
+      rt_cur = rt_next;
+      i += 2;
+    } else {
+      hipLaunchKernelGGL(k1, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
+                         i == inner ? 1 : 0, want_rrn, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
+                         b.diag, src, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r, b.r_store,
+                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2), ctl, dir);
+      if (pairs || (i & 1) == 0) dir = -dir;
+      i += 1;
+      prof_end(2, ev);
+      sd = 1 - sd;
+      continue;
+    }
+    prof_end(2, ev);
+    sd = 1 - sd;
+    dir = -dir;
+  }
+  TL_HIP(hipGetLastError());
+  return TL_OK;
+}
 
 // CG iterations (tea_solve.f90:341-369); see tl_cg_loop.  pin_first, when
 // given, replaces p[pcur] as the direction of the first iteration.
@@ -504,7 +632,6 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   const bool nt = use_nontemporal();
   auto kw = nt ? k_loop_w<true, T, V> : k_loop_w<false, T, V>;
   auto kur = nt ? k_pp_ur<true, T, V> : k_pp_ur<false, T, V>;
-  auto ksw = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
   const int c0 = *b.pcur;
   int sd = *b.sdcur;
   Ctl *ctl = (Ctl *)s.ctl;
@@ -514,7 +641,6 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   lp.check_switch = 0;
   lp.presteps = 0;
   const T theta_r = (T)(1.0 / theta);
-  const T *src_sd = b.diag ? b.z : b.r;
   auto launch = [&](int n, bool first) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
     const T *pin = (first && pin_first) ? pin_first : b.p[cin];
@@ -529,18 +655,8 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
                        b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag, part(P1),
                        part(P2), nb, ctl, h_rro(), h_alpha());
     prof_end(6, ev);
-    for (int i = 1; i <= inner; ++i) {
-      prof_begin(2, &ev);
-      hipLaunchKernelGGL(ksw, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
-                         i == inner ? 1 : 0, 1, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
-                         b.diag, src_sd, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r, b.r_store,
-                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2), ctl,
-                         (i & 1) ? 1 : -1);
-      prof_end(2, ev);
-      sd = 1 - sd;
-    }
-    TL_HIP(hipGetLastError());
-    return TL_OK;
+    T *rt_cur = b.rtemp;
+    return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, ctl, sd, rt_cur, 1);
   };
   auto tail = [&](int n) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
@@ -560,7 +676,8 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   const int executed = res.n_end - st->n_first + 1;  // outer iterations that ran
   if (inner_iters) *inner_iters += executed * inner;
   *b.pcur = c0 ^ ((executed + 1) & 1);               // + the stopping launch's p update
-  *b.sdcur = *b.sdcur ^ ((executed * inner) & 1);
+  const int passes = sweep_pairs() ? inner / 2 + inner % 2 : inner;
+  *b.sdcur = *b.sdcur ^ ((executed * passes) & 1);
   return TL_OK;
 }
 
@@ -570,23 +687,12 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
 template <typename T, int V>
 int run_ppcg_precondition(const LoopBufs<T> &b, const double *alphas, const double *betas,
                           int inner, double theta) {
-  State &s = S();
-  const Geo G = b.G;
-  const int nb = abs_blocks(G);
-  const bool nt = use_nontemporal();
-  auto ksw = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
   int sd = *b.sdcur;
-  const T theta_r = (T)(1.0 / theta);
-  for (int i = 1; i <= inner; ++i) {
-    hipLaunchKernelGGL(ksw, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
-                       i == inner ? 1 : 0, 0, theta_r, (T)alphas[i - 1], (T)betas[i - 1], b.diag,
-                       b.diag ? b.z : b.r, b.sd[sd], b.sd[1 - sd], b.rtemp, b.utemp, b.r,
-                       b.r_store, b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2),
-                       (const Ctl *)nullptr, (i & 1) ? 1 : -1);
-    sd = 1 - sd;
-  }
+  T *rt_cur = b.rtemp;
+  const int rc = queue_sweeps<T, V>(b, alphas, betas, inner, (T)(1.0 / theta), 0,
+                                    (const Ctl *)nullptr, sd, rt_cur, 1);
+  if (rc) return rc;
   *b.sdcur = sd;
-  TL_HIP(hipGetLastError());
   return TL_OK;
 }
 

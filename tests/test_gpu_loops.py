@@ -206,3 +206,42 @@ def test_cheby_run(pair_factory):
            b.ctypes.data_as(C.POINTER(C.c_double)), 1, 20, 1, C.byref(rr))
     assert_scalar(rr.value, rr_o, 1e-8, "cheby residual")
     p.cmp("u", tol=1e-10)
+
+
+@pytest.mark.parametrize("precond,reflective,inner,ny", [(1, False, 6, 70), (2, False, 7, 70),
+                                                          (1, True, 5, 70), (1, False, 4, 1400)])
+def test_ppcg_sweep_pairs_bit_identical(precond, reflective, inner, ny, monkeypatch):
+    """Two inner sweeps per pass (temporal blocking with ghost rows and
+    columns) compute what two single passes compute: the same operations on
+    the same operands, so any difference is the compiler's choice of fused
+    multiply-adds (last-bit level); a wrong ghost cell would show as O(1)."""
+    results = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("TL_SWEEP_PAIRS", pairs)
+        p = Pair(1100, ny, 1, seed=41, precond=precond, reflective=reflective)
+        try:
+            r0, rro = _setup(p)
+            ch_a, ch_b, theta = O.calc_ch_coefs(0.9, 40.0, inner)
+            a = np.ascontiguousarray(ch_a)
+            b = np.ascontiguousarray(ch_b)
+            pa, pb = a.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double))
+            L.call("tl_set_ppcg_active", 1)
+            p.d.scalar("tl_ppcg_init", 2)
+            L.call("tl_ppcg_init_sd", theta)
+            L.call("tl_ppcg_inner", pa, pb, 1, inner)
+            L.call("tl_ppcg_update_z")
+            rro_d = p.d.scalar("tl_ppcg_init", 3)
+            st = L.LoopState(n_first=2, n_last=40, eps_abs=1e-12 * math.sqrt(r0), rro=rro_d,
+                             old_error=0.0)
+            it = C.c_int(0)
+            L.call("tl_ppcg_loop", C.byref(st), pa, pb, inner, theta, C.byref(it))
+            results.append((st.n_end, st.error, [p.d.download(f) for f in ("u", "z", "r", "p")]))
+        finally:
+            p.close()
+    (n0, e0, f0), (n1, e1, f1) = results
+    assert n0 == n1
+    assert abs(e0 - e1) <= 1e-12 * abs(e0), (e0, e1)
+    for name, a, b in zip(("u", "z", "r", "p"), f0, f1):
+        d = np.max(np.abs(a - b)) / np.max(np.abs(a))
+        print(f"  pairs vs single sweeps: {name} max rel diff {d:.1e}", flush=True)
+        assert d <= 1e-12, (name, d)
