@@ -135,6 +135,17 @@ constexpr bool diff_form() {
 #ifndef TL_LDSX_SWEEP
 #define TL_LDSX_SWEEP 1
 #endif
+// minimum waves per SIMD the w pass is compiled for
+#ifndef TL_W_WAVES
+#define TL_W_WAVES 1
+#endif
+// minimum waves per SIMD the two-sweep pass is compiled for
+#ifndef TL_PAIR_WAVES64
+#define TL_PAIR_WAVES64 4
+#endif
+#ifndef TL_PAIR_WAVES32
+#define TL_PAIR_WAVES32 4
+#endif
 
 // Scalar head of the first kernel of an iteration: beta of iteration n-1,
 // exit test (tea_solve.f90:402-410) and, for CG, the switch test (:167).
@@ -184,7 +195,7 @@ __device__ inline Head loop_head(int n, int first, int tail, const LoopParams &l
 // On the launch that stops the loop only the p update is done, so p ends as
 // the reference leaves it.
 template <bool NT, typename T, int V>
-__global__ __launch_bounds__(BLOCK) void k_loop_w(Geo G, int n, int first, int tail,
+__global__ __launch_bounds__(BLOCK, TL_W_WAVES) void k_loop_w(Geo G, int n, int first, int tail,
                                                    LoopParams lp, double rro_in,
                                                    const T *__restrict__ src,
                                                    const T *__restrict__ pin,
@@ -416,7 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
 // (r on the first sweep), utemp; writes sdout, rtout, utemp.  first/last as
 // in k_pp_sweep, applying to sweep i / sweep i+1.
 template <bool NT, typename T, int V>
-__global__ __launch_bounds__(BLOCK) void k_pp_sweep2(
+__global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_WAVES64) void k_pp_sweep2(
     Geo G, int first_sweep, int last_sweep, int want_rrn, T theta_r, T a1, T b1, T a2, T b2,
     int diag, const T *__restrict__ src, const T *__restrict__ sdin, T *__restrict__ sdout,
     const T *__restrict__ rtin, T *__restrict__ rtout, T *__restrict__ utemp,

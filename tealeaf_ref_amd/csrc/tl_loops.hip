@@ -99,8 +99,12 @@ __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev
 #define TL_PF_CHEBY 0
 #endif
 
+#ifndef TL_CHEBY_WAVES
+#define TL_CHEBY_WAVES 1
+#endif
+
 template <bool NT>
-__global__ __launch_bounds__(BLOCK) void k_cheby_v(Geo G, const double *__restrict__ uin,
+__global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_v(Geo G, const double *__restrict__ uin,
                                                     double *__restrict__ uout,
                                                     const double *__restrict__ u0,
                                                     const double *__restrict__ Kx,
