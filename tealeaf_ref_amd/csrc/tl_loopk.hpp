@@ -126,9 +126,6 @@ constexpr bool diff_form() {
 #ifndef TL_ST_NT_SWEEP
 #define TL_ST_NT_SWEEP 0
 #endif
-#ifndef TL_UR_REV
-#define TL_UR_REV 1
-#endif
 #ifndef TL_LDSX_W
 #define TL_LDSX_W 1
 #endif
@@ -263,7 +260,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
                                                   const double *__restrict__ prev_part, int nprev,
                                                   double *__restrict__ out_part, Ctl *ctl,
                                                   const double *__restrict__ hrro,
-                                                  double *__restrict__ halpha) {
+                                                  double *__restrict__ halpha, int rev) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
@@ -278,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
     if (tt.c >= G.nx) continue;
     for (int k = tt.y0; k < tt.y1; ++k) {
-      const int y = TL_UR_REV ? tt.y1 - 1 - (k - tt.y0) : k;
+      const int y = rev ? tt.y1 - 1 - (k - tt.y0) : k;
       const long i = (long)y * P + tt.c;
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
       const VT rv = dev::ldv<NT, T, V>(r + i) - a * dev::ldv<NT, T, V>(w + i);
@@ -313,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double
                                                   const double *__restrict__ part_pw,
                                                   const double *__restrict__ part_zr, int nprev,
                                                   Ctl *ctl, double *__restrict__ hrro,
-                                                  double *__restrict__ halpha) {
+                                                  double *__restrict__ halpha, int rev) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
@@ -331,7 +328,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
     if (tt.c >= G.nx) continue;
     for (int k = tt.y0; k < tt.y1; ++k) {
-      const int y = TL_UR_REV ? tt.y1 - 1 - (k - tt.y0) : k;
+      const int y = rev ? tt.y1 - 1 - (k - tt.y0) : k;
       const long i = (long)y * P + tt.c;
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
       const VT ro = dev::ldv<NT, T, V>(r + i);
@@ -599,7 +596,7 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
     prof_end(0, ev);
     prof_begin(1, &ev);
     hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
-                       b.w, b.z, b.Mi, b.diag, part(P1), nb, part(P2), ctl, h_rro(), h_alpha());
+                       b.w, b.z, b.Mi, b.diag, part(P1), nb, part(P2), ctl, h_rro(), h_alpha(), 1);
     prof_end(1, ev);
     TL_HIP(hipGetLastError());
     return TL_OK;
@@ -652,22 +649,27 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   lp.check_switch = 0;
   lp.presteps = 0;
   const T theta_r = (T)(1.0 / theta);
+  // Row directions: the w pass walks d, the update -d, the sweep passes
+  // alternate from d; with an odd number of sweep passes the next outer
+  // iteration flips d, so every pass starts on the rows written last.
+  const int passes = inner / 2 + inner % 2;
   auto launch = [&](int n, bool first) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
     const T *pin = (first && pin_first) ? pin_first : b.p[cin];
+    const int d = ((passes & 1) && ((n - st->n_first) & 1)) ? -1 : 1;
     hipEvent_t ev;
     prof_begin(5, &ev);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
                        rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3),
-                       nb, part(P1), ctl, h_rro(), h_beta(), 1);
+                       nb, part(P1), ctl, h_rro(), h_beta(), d);
     prof_end(5, ev);
     prof_begin(6, &ev);
     hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, rro_first,
                        b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag, part(P1),
-                       part(P2), nb, ctl, h_rro(), h_alpha());
+                       part(P2), nb, ctl, h_rro(), h_alpha(), d > 0 ? 1 : 0);
     prof_end(6, ev);
     T *rt_cur = b.rtemp;
-    return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, ctl, sd, rt_cur, 1);
+    return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, ctl, sd, rt_cur, d);
   };
   auto tail = [&](int n) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
@@ -687,8 +689,8 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   const int executed = res.n_end - st->n_first + 1;  // outer iterations that ran
   if (inner_iters) *inner_iters += executed * inner;
   *b.pcur = c0 ^ ((executed + 1) & 1);               // + the stopping launch's p update
-  const int passes = sweep_pairs() ? inner / 2 + inner % 2 : inner;
-  *b.sdcur = *b.sdcur ^ ((executed * passes) & 1);
+  const int swaps = sweep_pairs() ? passes : inner;
+  *b.sdcur = *b.sdcur ^ ((executed * swaps) & 1);
   return TL_OK;
 }
 
