@@ -93,6 +93,28 @@ int tl_download(int field_id, double *host, int host_halo);
 /* Block until all queued work is done. */
 int tl_synchronize(void);
 
+/* ---- set-up on the device (initialise_chunk + generate_chunk) ----------- */
+#define TL_GEOM_RECTANGLE 1   /* g_rect, g_circ, g_point of data.f90:66-68 */
+#define TL_GEOM_CIRCLE 2
+#define TL_GEOM_POINT 3
+
+typedef struct {
+  int geometry;               /* TL_GEOM_*; ignored for the first state; any
+                               * other value covers no cell                  */
+  double density, energy;
+  double xmin, xmax, ymin, ymax, radius;  /* as read_input.f90 leaves them   */
+} tl_state;
+
+/* density, energy0 and u = energy0*density over the haloed chunk from the
+ * deck's states (states[0] = state 1 fills everything, later states
+ * overwrite where their geometry hits), and volume = dx*dy, computed on the
+ * device with the reference's vertex and cell-centre coordinates
+ * x = xmin + dx*(j-1) and 0.5*(x_j + x_{j+1}) rounded as there.
+ * Replaces initialise_chunk (kernels/initialise_chunk_kernel.f90:26-132)
+ * and generate_chunk (kernels/generate_chunk_kernel.f90:30-142). */
+int tl_generate(double xmin, double ymin, double dx, double dy, int nstates,
+                const tl_state *states);
+
 /* ---- per-step kernels (tea_leaf_common.f90) ------------------------------ */
 
 /* u = u0 = energy1*density; Kx, Ky from density or 1/density; zero-flux

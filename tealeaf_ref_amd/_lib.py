@@ -38,6 +38,12 @@ class LoopState(C.Structure):
                 ("reason", C.c_int), ("error", C.c_double)]
 
 
+class TLState(C.Structure):
+    _fields_ = [("geometry", C.c_int), ("density", C.c_double), ("energy", C.c_double),
+                ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
+                ("ymax", C.c_double), ("radius", C.c_double)]
+
+
 class MixedState(C.Structure):
     _fields_ = [("solver", C.c_int), ("max_outer", C.c_int), ("max_inner", C.c_int),
                 ("inner_steps", C.c_int), ("presteps", C.c_int), ("rel_tol", C.c_double),
@@ -77,6 +83,7 @@ def lib():
             "tl_jacobi_loop": [P(LoopState)],
             "tl_cheby_run": [PD, PD, I, I, I, PD],
             "tl_mixed_solve": [P(MixedState)],
+            "tl_generate": [D, D, D, D, I, P(TLState)],
             "tl_snapshot_reserve": [I], "tl_snapshot_save": [I, I], "tl_snapshot_load": [I, I],
             "tl_snapshot_fill": [I, D], "tl_snapshot_upload": [I, PD, I],
             "tl_snapshot_download": [I, PD, I], "tl_adjoint_face_grad": [I, I, I, I],

@@ -61,6 +61,7 @@ class TeaLeafSteps:
         self.mixed_inner_tol = mixed_inner_tol
         self.reflective = bool(reflective)
         self.iterations = 0            # solver iterations of the last forward/backward
+        self.progress = None           # optional callable(str), told of every step
         self.dev = L.Device(self.nx, self.ny, 1, device)
         self.dev.set_options(coefficient, L.PREC_NONE, self.reflective)
         self.dev.upload("energy1", _pad(e0))
@@ -118,6 +119,8 @@ class TeaLeafSteps:
             if (n - 1) % self.k == 0:
                 L.call("tl_snapshot_save", self.CK + (n - 1) // self.k, L.FIELD["energy1"])
             self._step()
+            if self.progress:
+                self.progress(f"forward step {n}/{self.steps}, {self.iterations} iterations")
         self._forward_done = True
         return self.dev.download("u")[1:-1, 1:-1].copy()
 
@@ -149,6 +152,8 @@ class TeaLeafSteps:
                 L.call("tl_adjoint_face_grad", L.FIELD["u"], self.SEG + (n - a - 1),
                        self.E0 if n == 1 else -1, self.GRAD)
                 L.call("tl_snapshot_save", self.RHS, L.FIELD["u"])   # g_{n-1} = lambda_n
+                if self.progress:
+                    self.progress(f"adjoint step {n}, {self.iterations} iterations")
         out = np.zeros((self.ny + 2, self.nx + 2))
         L.call("tl_snapshot_download", self.GRAD, out.ctypes.data_as(C.POINTER(C.c_double)), 1)
         return out[1:-1, 1:-1].copy()

@@ -507,67 +507,26 @@ CONTAINS
   ! ===========================================================================
   ! setup: geometry, state painting, upload (start.f90 on one chunk)
   ! ===========================================================================
+  !> initialise_chunk + generate_chunk on the device (tl_generate): density,
+  !> energy0, u = energy0*density and volume from the deck's states.
   SUBROUTINE generate_and_upload()
-    REAL(KIND=8), ALLOCATABLE :: vx(:), vy(:), cx(:), cy(:), density(:,:), energy0(:,:), u(:,:), vol(:,:)
-    INTEGER :: j, k, s, h
-    REAL(KIND=8) :: rad, xc, yc
-    LOGICAL :: hit
-    h = halo_depth
+    TYPE(tl_state), ALLOCATABLE :: ts(:)
+    INTEGER :: s
     dx = (xmax - xmin) / REAL(x_cells)
     dy = (ymax - ymin) / REAL(y_cells)
-    ALLOCATE(vx(-1:x_cells + 3), vy(-1:y_cells + 3), cx(-1:x_cells + 2), cy(-1:y_cells + 2))
-    DO j = -1, x_cells + 3
-      vx(j) = xmin + dx * REAL(j - 1)
+    ALLOCATE(ts(number_of_states))
+    DO s = 1, number_of_states
+      ts(s)%geometry = INT(states(s)%geometry, c_int)
+      ts(s)%density = states(s)%density
+      ts(s)%energy = states(s)%energy
+      ts(s)%xmin = states(s)%xmin
+      ts(s)%xmax = states(s)%xmax
+      ts(s)%ymin = states(s)%ymin
+      ts(s)%ymax = states(s)%ymax
+      ts(s)%radius = states(s)%radius
     ENDDO
-    DO k = -1, y_cells + 3
-      vy(k) = ymin + dy * REAL(k - 1)
-    ENDDO
-    DO j = -1, x_cells + 2
-      cx(j) = 0.5 * (vx(j) + vx(j + 1))
-    ENDDO
-    DO k = -1, y_cells + 2
-      cy(k) = 0.5 * (vy(k) + vy(k + 1))
-    ENDDO
-    ALLOCATE(vol(-1:x_cells + 2, -1:y_cells + 2))
-    vol = dx * dy
-    CALL hip_check(tl_upload(TLF_VOLUME, vol, 2_c_int), 'start')
-    DEALLOCATE(vol)
-
-    ALLOCATE(density(1 - h:x_cells + h, 1 - h:y_cells + h), energy0(1 - h:x_cells + h, 1 - h:y_cells + h))
-    energy0 = states(1)%energy
-    density = states(1)%density
-    DO s = 2, number_of_states
-      xc = states(s)%xmin
-      yc = states(s)%ymin
-      DO k = 1 - h, y_cells + h
-        DO j = 1 - h, x_cells + h
-          hit = .FALSE.
-          SELECT CASE (states(s)%geometry)
-          CASE (g_rect)
-            IF (j >= 1 .AND. j <= x_cells .AND. k >= 1 .AND. k <= y_cells) THEN
-              hit = vx(j + 1) >= states(s)%xmin .AND. vx(j) < states(s)%xmax .AND. &
-                    vy(k + 1) >= states(s)%ymin .AND. vy(k) < states(s)%ymax
-            ENDIF
-          CASE (g_circ)
-            rad = SQRT((cx(MIN(MAX(j, -1), x_cells + 2)) - xc) * (cx(MIN(MAX(j, -1), x_cells + 2)) - xc) + &
-                       (cy(MIN(MAX(k, -1), y_cells + 2)) - yc) * (cy(MIN(MAX(k, -1), y_cells + 2)) - yc))
-            hit = rad <= states(s)%radius
-          CASE (g_point)
-            hit = vx(MIN(MAX(j, -1), x_cells + 3)) == xc .AND. vy(MIN(MAX(k, -1), y_cells + 3)) == yc
-          END SELECT
-          IF (hit) THEN
-            energy0(j, k) = states(s)%energy
-            density(j, k) = states(s)%density
-          ENDIF
-        ENDDO
-      ENDDO
-    ENDDO
-    ALLOCATE(u(1 - h:x_cells + h, 1 - h:y_cells + h))
-    u = energy0 * density
-    CALL hip_check(tl_upload(TLF_DENSITY, density, INT(h, c_int)), 'start')
-    CALL hip_check(tl_upload(TLF_ENERGY0, energy0, INT(h, c_int)), 'start')
-    CALL hip_check(tl_upload(TLF_U, u, INT(h, c_int)), 'start')
-    DEALLOCATE(vx, vy, cx, cy, density, energy0, u)
+    CALL hip_check(tl_generate(xmin, ymin, dx, dy, INT(number_of_states, c_int), ts), 'start')
+    DEALLOCATE(ts)
   END SUBROUTINE generate_and_upload
 
   ! ===========================================================================

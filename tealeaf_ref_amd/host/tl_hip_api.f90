@@ -27,6 +27,11 @@ MODULE tl_hip_api
 
   INTEGER(c_int), PARAMETER :: TL_MIXED_CG = 1, TL_MIXED_PPCG = 2
 
+  TYPE, BIND(C) :: tl_state
+    INTEGER(c_int) :: geometry
+    REAL(c_double) :: density, energy, xmin, xmax, ymin, ymax, radius
+  END TYPE tl_state
+
   TYPE, BIND(C) :: tl_mixed_state
     INTEGER(c_int) :: solver, max_outer, max_inner, inner_steps, presteps
     REAL(c_double) :: rel_tol, inner_tol, eigmin, eigmax
@@ -141,6 +146,12 @@ MODULE tl_hip_api
       INTEGER(c_int), VALUE :: inner
       REAL(c_double), VALUE :: theta
       INTEGER(c_int), INTENT(INOUT) :: inner_iters
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_generate(xmin, ymin, dx, dy, nstates, states) BIND(C, name="tl_generate")
+      IMPORT :: c_int, c_double, tl_state
+      REAL(c_double), VALUE :: xmin, ymin, dx, dy
+      INTEGER(c_int), VALUE :: nstates
+      TYPE(tl_state), INTENT(IN) :: states(*)
     END FUNCTION
     INTEGER(c_int) FUNCTION tl_mixed_solve(st) BIND(C, name="tl_mixed_solve")
       IMPORT :: c_int, tl_mixed_state
