@@ -124,6 +124,21 @@ def solver_runs(cells, steps, warmup):
     return out
 
 
+def small_run(steps=10):
+    """The reference's CPU-sized case: 512x512, 3 seeded rectangles (seed 1),
+    FP64 CG, tl_eps 1e-15, 10 steps; seconds per step and per iteration."""
+    from tealeaf_ref_amd.deckgen import make_deck
+    deck = make_deck(512, solver="cg", seed=1, eps=1e-15, max_iters=5000, end_step=steps + 1)
+    with tempfile.TemporaryDirectory() as wd:
+        elapsed, rows, _ = run_deck_timed(deck, steps, 1, wd)
+    its = sum(r["iterations"] + 1 for r in rows)
+    rec = {"workload": "512x512 FP64 CG, seeded deck (3 rectangles, seed 1), tl_eps 1e-15",
+           "s_per_step": elapsed / steps, "iterations_per_step": its / steps,
+           "us_per_iteration": elapsed / its * 1e6}
+    log(f"[bench] small: {rec}")
+    return rec
+
+
 def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
     """Run `warmup` + `steps` timesteps of a deck through the native host;
     returns the timed seconds, the per-step results of the timed steps and
@@ -166,7 +181,7 @@ def precision_comparison(cells, steps, warmup, inner_steps=10):
     from tealeaf_ref_amd.deckgen import bench_deck
     import numpy as np
     runs = {}
-    base = dict(coefficient="inverse_density", eps=1e-10, max_iters=20000,
+    base = dict(coefficient="inverse_density", eps=1e-10, max_iters=100000,
                 end_step=steps + warmup, summary_frequency=1, check_result=True)
     plan = [("fp64_cg", "cg", ()), ("fp64_ppcg", "ppcg", ()),
             ("mixed_cg", "cg", ("tl_mixed_precision",)),
@@ -255,6 +270,8 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
         p = TeaLeafSteps(e0, rx, ry, steps, coefficient=_lib.CONDUCTIVITY,
                          eps=1e-15 if prec == "fp64" else mixed_eps, checkpoint_every=every,
                          precision=prec)
+        if cells >= 4096:
+            p.progress = lambda msg: log(f"[bench] adjoint {prec} {cells}^2: {msg}")
         target = p.forward(theta_p)
         t0 = time.perf_counter()
         u = p.forward(theta)
@@ -391,6 +408,7 @@ def main():
     ap.add_argument("--adjoint-precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
                     help="true relative residual of the mixed-precision solves")
+    ap.add_argument("--no-small", action="store_true", help="skip the 512^2 CG case")
     ap.add_argument("--solver-steps", type=int, default=1,
                     help="timed steps of the Chebyshev and PPCG runs (0 to skip)")
     ap.add_argument("--mixed-steps", type=int, default=2)
@@ -448,6 +466,8 @@ def main():
     }
     if args.solver_steps:
         out["solvers"] = solver_runs(args.cells, args.solver_steps, 1)
+    if not args.no_small:
+        out["small"] = small_run()
     sizes = [int(x) for x in args.mixed_sizes.split(",") if x.strip()]
     if sizes:
         out["precision_comparison"] = [
