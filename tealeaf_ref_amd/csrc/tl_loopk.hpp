@@ -298,6 +298,69 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
   if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
 }
 
+// CG iteration n, second pass with the block-Jacobi preconditioner:
+// u += alpha p, r -= alpha w over groups of 4 rows, then z = M^-1 r by the
+// 4-row Thomas solve of tea_block_solve (kernels/tea_leaf_common_kernel.f90:
+// 378-468) and the partials of r.z.  Tiles start on multiples of 4 rows.
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_cg_ur_blk(
+    Geo G, int n, T *__restrict__ u, const T *__restrict__ p, T *__restrict__ r,
+    const T *__restrict__ w, T *__restrict__ z, const T *__restrict__ Di,
+    const T *__restrict__ Ky, const T *__restrict__ cp, const T *__restrict__ bfp, T ry,
+    const double *__restrict__ prev_part, int nprev, double *__restrict__ out_part, Ctl *ctl,
+    const double *__restrict__ hrro, double *__restrict__ halpha, int rev) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl->stop) return;
+  const double pw = dev::sum_partials(prev_part, nprev, lds);
+  const double alpha = hrro[n] / pw;
+  if (blockIdx.x == 0 && threadIdx.x == 0) halpha[n] = alpha;
+  const T a = (T)alpha;
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    if (tt.c >= G.nx) continue;
+    const int ngroups = (tt.y1 - tt.y0 + 3) / 4;
+    for (int m = 0; m < ngroups; ++m) {
+      const int g0 = tt.y0 + 4 * (rev ? ngroups - 1 - m : m);
+      const int nq = min(4, G.ny - g0);
+      VT rr[4], dp[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q >= nq) break;
+        const long i = (long)(g0 + q) * P + tt.c;
+        const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
+        rr[q] = dev::ldv<NT, T, V>(r + i) - a * dev::ldv<NT, T, V>(w + i);
+        dev::stv<false, T, V>(u + i - tt.c, tt.c, G.nx, uv);
+        dev::stv<false, T, V>(r + i - tt.c, tt.c, G.nx, rr[q]);
+      }
+      const long i0 = (long)g0 * P + tt.c;
+      dp[0] = rr[0] / dev::ldv<NT, T, V>(Di + i0);
+#pragma unroll
+      for (int q = 1; q < 4; ++q) {
+        if (q >= nq) break;
+        const long i = i0 + (long)q * P;
+        dp[q] = (rr[q] - (-dev::ldv<NT, T, V>(Ky + i) * ry) * dp[q - 1]) *
+                dev::ldv<NT, T, V>(bfp + i);
+      }
+      VT zn = dp[nq - 1];
+      for (int q = nq - 1; q >= 0; --q) {
+        const long i = i0 + (long)q * P;
+        const VT zq = q == nq - 1 ? dp[q] : dp[q] - dev::ldv<NT, T, V>(cp + i) * zn;
+        dev::stv<false, T, V>(z + i - tt.c, tt.c, G.nx, zq);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (tt.c + e < G.nx) acc += (double)rr[q][e] * (double)zq[e];
+        zn = zq;
+      }
+    }
+  }
+  acc = dev::block_sum(acc, lds);
+  if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+}
+
 // PPCG outer iteration n, second pass (tea_solve.f90:300-309): rro = z.r
 // (partials from the previous iteration's last sweep, or rro_in on the
 // first), alpha = rro/pw; u += alpha p; r_store = r; r -= alpha w; (z = Mi r)
@@ -506,6 +569,10 @@ struct LoopBufs {
   T rx, ry;
   int diag;               // Jacobi-diagonal preconditioner
   const T *cg_src;        // r, or z when preconditioned: the CG direction source
+  // block-Jacobi preconditioner (CG only): Di, Thomas coefficients; G's tiles
+  // then start on multiples of 4 rows
+  int block = 0;
+  const T *Di = nullptr, *cp = nullptr, *bfp = nullptr;
 };
 
 inline int abs_blocks(const Geo &G) { return G.blocks < 0 ? -G.blocks : G.blocks; }
@@ -578,6 +645,7 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
   const bool nt = use_nontemporal();
   auto kw = nt ? k_loop_w<true, T, V> : k_loop_w<false, T, V>;
   auto kur = nt ? k_cg_ur<true, T, V> : k_cg_ur<false, T, V>;
+  auto kub = nt ? k_cg_ur_blk<true, T, V> : k_cg_ur_blk<false, T, V>;
   const int c0 = *b.pcur;
   LoopParams lp;
   lp.eps_abs = st->eps_abs;
@@ -595,8 +663,14 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
                        part(P2), nb, part(P1), ctl, h_rro(), h_beta(), 1);
     prof_end(0, ev);
     prof_begin(1, &ev);
-    hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
-                       b.w, b.z, b.Mi, b.diag, part(P1), nb, part(P2), ctl, h_rro(), h_alpha(), 1);
+    if (b.block)
+      hipLaunchKernelGGL(kub, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
+                         b.w, b.z, b.Di, b.Ky, b.cp, b.bfp, b.ry, part(P1), nb, part(P2), ctl,
+                         h_rro(), h_alpha(), 1);
+    else
+      hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
+                         b.w, b.z, b.Mi, b.diag, part(P1), nb, part(P2), ctl, h_rro(), h_alpha(),
+                         1);
     prof_end(1, ev);
     TL_HIP(hipGetLastError());
     return TL_OK;

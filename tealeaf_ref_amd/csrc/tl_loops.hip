@@ -236,7 +236,20 @@ LoopBufs<double> fp64_bufs() {
   b.rx = s.rx;
   b.ry = s.ry;
   b.diag = s.precond == TL_PREC_JAC_DIAG;
-  b.cg_src = (b.diag || s.ppcg_active) ? s.z.p11 : s.r.p11;
+  b.block = s.precond == TL_PREC_JAC_BLOCK;
+  b.Di = s.Di.p11;
+  b.cp = s.cp.p11;
+  b.bfp = s.bfp.p11;
+  b.cg_src = (b.diag || b.block || s.ppcg_active) ? s.z.p11 : s.r.p11;
+  if (b.block) {
+    // tiles start on the 4-row blocks of the preconditioner
+    Geo &G = b.G;
+    const int blocks = G.blocks < 0 ? -G.blocks : G.blocks;
+    G.rows_per_tile = (G.rows_per_tile + 3) / 4 * 4;
+    G.tiles = (long)G.tiles_x * ((G.ny + G.rows_per_tile - 1) / G.rows_per_tile);
+    const int nb = (int)(G.tiles < blocks ? G.tiles : blocks);
+    G.blocks = G.blocks < 0 ? -nb : nb;
+  }
   return b;
 }
 
@@ -317,9 +330,9 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
   int rc = check_state(st);
   if (rc) return rc;
   State &s = S();
-  if (s.precond == TL_PREC_JAC_BLOCK || !s.zb_all) {
-    // block-tridiagonal preconditioner, or a chunk with internal faces: the
-    // per-routine path, one iteration at a time
+  if (!s.zb_all) {
+    // a chunk with internal faces: the per-routine path, one iteration at a
+    // time
     double rro = st->rro, old_error = st->old_error, err = 0.0;
     for (int n = st->n_first; n <= st->n_last; ++n) {
       if (n > st->n_first && st->check_switch && fabs(old_error) <= st->epslim &&
