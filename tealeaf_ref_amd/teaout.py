@@ -7,7 +7,9 @@ and eigen lines tea_solve.f90:216-220; ``est itc`` tea_leaf_cheby.f90:245-250.
 """
 import re
 
-_FLOAT = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[EeDd][-+]?\d+)?"
+# numbers as Fortran writes them, including NaN and Infinity (a one-cell mesh
+# has a zero residual and produces 0/0 in the reference as here)
+_FLOAT = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[EeDd][-+]?\d+)?|[-+]?NaN|[-+]?Infinity|[-+]?Inf"
 _STEP_HDR = re.compile(r"^\s*Step\s+(\d+)\s+time\s+(" + _FLOAT + r")\s+timestep\s+(" + _FLOAT + ")")
 _SUMMARY = re.compile(r"^\s*step:\s*(\d+)\s+(" + _FLOAT + r")\s+(" + _FLOAT + r")\s+(" + _FLOAT +
                       r")\s+(" + _FLOAT + r")\s+(" + _FLOAT + r")")
