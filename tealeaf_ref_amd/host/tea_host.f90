@@ -708,7 +708,9 @@ CONTAINS
     INTEGER :: n, k, m, est_itc, cheby_calc_steps, max_cheby_iters, ppcg_inner_iters, info, nc
     LOGICAL :: ch_switch_check, exited, is_check
     REAL(KIND=8) :: old_error, error, exact_error, initial_residual, rro, bb, e_sim, norm
-    REAL(KIND=8) :: it_alpha, gamm, t_total, t_init, t_solve, t_reset
+    REAL(KIND=8) :: it_alpha, gamm, t_total, t_init, t_solve, t_reset, h0
+    REAL(KIND=8) :: t_ch, cg_time, ch_time, cg_per_it, ch_per_it
+    INTEGER :: cg_calc_steps
     INTEGER :: fields(NUM_FIELDS)
     INTEGER(c_int) :: zb(4), inner_c
     REAL(KIND=8), ALLOCATABLE :: cg_alphas(:), cg_betas(:), ch_alphas(:), ch_betas(:)
@@ -728,6 +730,8 @@ CONTAINS
     last_switch = 0
     last_est_itc = 0
     error = 0.0_8
+    cg_calc_steps = 0
+    t_ch = -1.0_8
 
     IF (coefficient /= RECIP_CONDUCTIVITY .AND. coefficient /= CONDUCTIVITY) &
       CALL report_error('tea_leaf', 'unknown coefficient option')
@@ -735,6 +739,7 @@ CONTAINS
 
     t_total = timer()
     t_init = t_total
+    h0 = profiler%halo_update   ! halo time is excluded from the windows (tea_solve.f90:93-95)
 
     fields = 0
     fields(FIELD_ENERGY1) = 1
@@ -759,8 +764,9 @@ CONTAINS
     last_refinements = 0
     IF (tl_mixed_precision) THEN
       ! the whole iteration loop runs in the library (tl_mixed_solve)
-      IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init)
+      IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init) - (profiler%halo_update - h0)
       t_solve = timer()
+      h0 = profiler%halo_update
       CALL mixed_solve(initial_residual, error, n, ppcg_inner_iters, exited)
     ELSE IF (tl_use_cg .OR. tl_use_chebyshev .OR. tl_use_ppcg) THEN
       CALL hip_check(tl_cg_init(rro), 'tea_leaf_cg_init')
@@ -776,8 +782,9 @@ CONTAINS
     ENDIF
 
     IF (.NOT. tl_mixed_precision) THEN
-      IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init)
+      IF (profiler_on) profiler%tea_init = profiler%tea_init + (timer() - t_init) - (profiler%halo_update - h0)
       t_solve = timer()
+      h0 = profiler%halo_update
       n = 1
       exited = .FALSE.
     ENDIF
@@ -794,6 +801,7 @@ CONTAINS
       ENDIF
 
       IF ((tl_use_chebyshev .OR. tl_use_ppcg) .AND. ch_switch_check) THEN
+        IF (t_ch < 0.0_8) t_ch = timer()         ! start of the Chebyshev phase
         IF (cheby_calc_steps == 0) THEN          ! :183-253
           max_cheby_iters = max_iters - n + 2
           IF (first) THEN
@@ -931,6 +939,8 @@ CONTAINS
         st%rro = rro
         st%old_error = old_error
         CALL hip_check(tl_cg_loop(st, cg_alphas, cg_betas), 'tea_leaf_cg')
+        ! CG iterations run (:343); a switch is found at the top of n_end
+        cg_calc_steps = cg_calc_steps + st%n_end - st%n_first + MERGE(0, 1, st%reason == TL_LOOP_SWITCH)
         rro = st%rro
         error = st%error
         SELECT CASE (st%reason)
@@ -963,6 +973,14 @@ CONTAINS
       ENDIF
     ENDDO outer
     IF (.NOT. exited .AND. .NOT. tl_mixed_precision) n = max_iters + 1
+    ! time of the CG and the Chebyshev iterations (tea_solve.f90:387-393)
+    IF (t_ch >= 0.0_8) THEN
+      ch_time = timer() - t_ch
+      cg_time = t_ch - t_solve
+    ELSE
+      ch_time = 0.0_8
+      cg_time = timer() - t_solve
+    ENDIF
 
     IF (tl_check_result) THEN                        ! :416-431
       fields = 0
@@ -973,7 +991,7 @@ CONTAINS
       exact_error = SQRT(exact_error)
     ENDIF
 
-    IF (profiler_on) profiler%tea_solve = profiler%tea_solve + (timer() - t_solve)
+    IF (profiler_on) profiler%tea_solve = profiler%tea_solve + (timer() - t_solve) - (profiler%halo_update - h0)
 
     WRITE(g_out, "('Conduction error ',e14.7)") error / initial_residual
     WRITE(0, "('Conduction error ',e14.7)") error / initial_residual
@@ -993,11 +1011,12 @@ CONTAINS
     last_cond_error = error / initial_residual
 
     t_reset = timer()
+    h0 = profiler%halo_update
     CALL hip_check(tl_finalise(), 'tea_leaf_finalise')
     fields = 0
     fields(FIELD_ENERGY1) = 1
     CALL update_halo(fields, 1)
-    IF (profiler_on) profiler%tea_reset = profiler%tea_reset + (timer() - t_reset)
+    IF (profiler_on) profiler%tea_reset = profiler%tea_reset + (timer() - t_reset) - (profiler%halo_update - h0)
 
     CALL hip_check(tl_synchronize(), 'tea_leaf')
     t_total = timer() - t_total
@@ -1005,6 +1024,25 @@ CONTAINS
     IF (profiler_on) THEN
       WRITE(0, "(a16,f16.10,a7,i7,a16,f16.10)") "Solve Time", t_total, "Its", n, "Time Per It", t_total / n
       WRITE(g_out, "(a16,f16.10,a7,i7,a16,f16.10)") "Solve Time", t_total, "Its", n, "Time Per It", t_total / n
+    ENDIF
+    IF (profiler_on .AND. tl_use_chebyshev .AND. .NOT. tl_mixed_precision) THEN   ! :478-501
+      cg_per_it = 0.0_8
+      ch_per_it = 0.0_8
+      IF (cg_calc_steps > 0) cg_per_it = cg_time / cg_calc_steps
+      IF (cheby_calc_steps > 0) ch_per_it = ch_time / cheby_calc_steps
+      DO k = 1, 2
+        m = MERGE(0, g_out, k == 1)
+        WRITE(m, "(a3, a16, a7, a16, a7)") "", "Time", "Its", "Per it", "Ratio"
+        WRITE(m, "(a3, f16.10, i7, f16.10, f7.2)") "CG", cg_time, cg_calc_steps, cg_per_it, 1.0_8
+        IF (cheby_calc_steps > 0) THEN
+          WRITE(m, "(a3, f16.10, i7, f16.10, f7.2)") "CH", ch_time, cheby_calc_steps, ch_per_it, &
+            ch_per_it / cg_per_it
+        ELSE
+          WRITE(m, "(a3, f16.10, i7, f16.10, f7.2)") "CH", ch_time, cheby_calc_steps, ch_per_it, 0.0_8
+        ENDIF
+        WRITE(m, "('Chebyshev actually took ', i6, ' (', i6, ' off guess)')") cheby_calc_steps, &
+          cheby_calc_steps - est_itc
+      ENDDO
     ENDIF
     DEALLOCATE(cg_alphas, cg_betas, ch_alphas, ch_betas)
   END SUBROUTINE tea_leaf
@@ -1081,6 +1119,7 @@ CONTAINS
     WRITE(g_out, '(a15,f8.3)') 'Tea Version ', g_version
     WRITE(g_out, '(a18)') 'HIP Version'
     WRITE(g_out, '(a14,i6)') 'Task Count ', 1
+    WRITE(g_out, '(a15,i5)') 'Thread Count: ', 1   ! one host thread drives the GPU
     WRITE(g_out, *)
     WRITE(g_out, *) 'Tea will run from the following input:-'
     WRITE(g_out, *)
@@ -1107,7 +1146,15 @@ CONTAINS
 
     WRITE(g_out, *) 'Setting up initial geometry'
     WRITE(g_out, *)
+    ! one chunk of one tile of one sub-tile (tea.f90:148-150, tea.f90:282-284, start.f90:79-91)
+    WRITE(g_out, *)
+    WRITE(g_out, *) "Mesh ratio of ", REAL(x_cells) / REAL(y_cells)
+    WRITE(g_out, *) "Decomposing the mesh into ", 1, " by ", 1, " chunks"
+    WRITE(g_out, *) "Decomposing each chunk into ", 1, " by ", 1, " tiles"
+    WRITE(g_out, *)
     WRITE(g_out, *) "Tile size ", x_cells, " by ", y_cells, " cells"
+    WRITE(g_out, *) "Sub-tile size ranges from ", x_cells, " by ", y_cells, " cells", &
+                    " to ", x_cells, " by ", y_cells, " cells"
     CALL hip_check(tl_init(INT(x_cells, c_int), INT(y_cells, c_int), INT(halo_depth, c_int), &
                            INT(device, c_int)), 'start')
     host_ready = .TRUE.
@@ -1127,6 +1174,8 @@ CONTAINS
     CALL field_summary()
     WRITE(g_out, *) 'Starting the calculation'
     complete = .FALSE.
+    ! the profile covers the timed run, so its total stays within the wall clock
+    profiler = profiler_type()
     timerstart = timer()
   END SUBROUTINE initialise
 

@@ -93,3 +93,17 @@ def last_summary_per_step(parsed):
     for s in parsed["summaries"]:
         out[s["step"]] = s
     return out
+
+
+_NUM_IN_LINE = re.compile(r"(?<![A-Za-z_\d.])(?:" + _FLOAT + r")(?![A-Za-z_])")
+
+
+def skeleton(text):
+    """The line layout of a tea.out: every number replaced by ``#`` and runs of
+    blanks collapsed, so two logs of the same deck compare equal line by line
+    whatever their values and timings."""
+    out = []
+    for line in text.splitlines():
+        line = _NUM_IN_LINE.sub("#", line)
+        out.append(" ".join(line.split()))
+    return out

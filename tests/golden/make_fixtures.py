@@ -3,7 +3,8 @@
 Runs oracle/_ref/tea_leaf (the unmodified reference built from its sources by
 oracle/ref.mk) on one thread for every seeded deck below, and stores the
 parsed tea.out (per-step `step:` rows, iteration counts, switch/eigen lines)
-as JSON under tests/golden/fixtures/.  Also converts the goldens shipped with
+as JSON under tests/golden/fixtures/; for the layout_* decks also the line
+layout of the whole tea.out (teaout.skeleton).  Also converts the goldens shipped with
 the reference (Benchmarks/tea_bm_N.{in,out}) into the same JSON form.
 
     python tests/golden/make_fixtures.py [name ...]
@@ -54,6 +55,15 @@ CASES = {
     "ppcg_jacblock_96": dict(cells=96, solver="ppcg", seed=13, precond="jac_block", inner_steps=6,
                              end_step=3),
     "cg_checkresult_64": dict(cells=64, solver="cg", seed=12, check_result=True, end_step=2),
+    # small decks with the profiler on whose whole tea.out layout is kept
+    "layout_cg_32": dict(cells=32, solver="cg", seed=14, precond="jac_diag", check_result=True,
+                         end_step=2, profiler=True),
+    "layout_cheby_64": dict(cells=64, solver="chebyshev", seed=14, eps=1e-12, max_iters=2000,
+                            end_step=2, profiler=True),
+    "layout_ppcg_64": dict(cells=64, solver="ppcg", seed=3, n_rect=2, n_circ=1, eps=1e-15,
+                           max_iters=500, inner_steps=4, end_step=2, profiler=True),
+    "layout_jacobi_32": dict(cells=32, solver="jacobi", seed=14, eps=1e-6, max_iters=4000,
+                             reflective=True, end_step=2, profiler=True),
 }
 
 
@@ -83,8 +93,11 @@ def main(names):
         cells = kw.pop("cells")
         deck = make_deck(cells, **kw)
         print("reference run:", name, flush=True)
-        fx = fixture_from(run_reference(deck), deck,
+        text = run_reference(deck)
+        fx = fixture_from(text, deck,
                           "oracle/_ref/tea_leaf (reference built from source), OMP_NUM_THREADS=1")
+        if name.startswith("layout_"):
+            fx["skeleton"] = teaout.skeleton(text)
         with open(os.path.join(OUT, name + ".json"), "w") as f:
             json.dump(fx, f, indent=1)
     # goldens shipped with the reference (data, converted)

@@ -1,0 +1,76 @@
+"""The tea.out the native program writes has the reference's layout line for
+line (numbers aside), and its profiler table adds up.
+
+The layout_* fixtures hold the line skeleton of the reference's own tea.out
+for small decks with the profiler on (oracle/_ref/tea_leaf, one thread).  The
+only lines allowed to differ name the build: the reference's "MPI Version" /
+"OpenMP Version" (initialise.f90:50-56) and "Using Fortran Kernels"
+(read_input.f90) against this program's "HIP Version" and "Using HIP kernels".
+"""
+import re
+import subprocess
+
+import pytest
+
+import fixtures_util as FX
+from tealeaf_ref_amd import host, teaout
+
+pytestmark = pytest.mark.gpu
+
+LAYOUT = [n for n in FX.names() if n.startswith("layout_")]
+PROFILE_ROWS = ("Timestep", "MPI Halo Exchange", "Self Halo Update", "Internal Halo Update",
+                "Summary", "Visit", "Tea Init", "Dot Product", "Tea Solve", "Tea Reset",
+                "Set Field", "Total", "The Rest")
+
+
+def _as_built_here(ref_lines):
+    """The reference's skeleton with its build-naming lines replaced by ours."""
+    out = []
+    for line in ref_lines:
+        if line == "OpenMP Version":
+            continue
+        if line == "MPI Version":
+            line = "HIP Version"
+        if line == "Using Fortran Kernels":
+            line = "Using HIP kernels (gfx950)"
+        out.append(line)
+    return out
+
+
+def _profile(text):
+    rows = {}
+    for line in text.splitlines():
+        m = re.match(r"^(\S.*?)\s+:\s+(\S+)\s+(\S+)\s*$", line)
+        if m and m.group(1).strip() in PROFILE_ROWS:
+            rows[m.group(1).strip()] = (float(m.group(2)), float(m.group(3)))
+    return rows
+
+
+def test_layout_fixtures_present():
+    assert len(LAYOUT) >= 4, LAYOUT
+
+
+@pytest.mark.parametrize("name", LAYOUT)
+def test_teaout_layout(name, tmp_path):
+    fx = FX.load(name)
+    print(f"[teaout] {name}", flush=True)
+    (tmp_path / "tea.in").write_text(fx["deck"])
+    subprocess.run([host.TEA_LEAF_BIN], cwd=tmp_path, check=True, timeout=120,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    text = (tmp_path / "tea.out").read_text()
+    got = teaout.skeleton(text)
+    want = _as_built_here(fx["skeleton"])
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert g == w, f"line {i + 1}: {g!r} vs reference {w!r}"
+    assert len(got) == len(want), (len(got), len(want), got[len(want):] or want[len(got):])
+    # same per-step rows and iteration counts as the reference
+    parsed = teaout.parse(text)
+    FX.compare_summaries(parsed["summaries"], fx["summaries"], 1e-10)
+    FX.compare_iterations(parsed["steps"], fx["steps"])
+    # the profile covers the timed run: parts within the wall clock, none negative
+    rows = _profile(text)
+    assert tuple(rows) == PROFILE_ROWS, tuple(rows)
+    for k, (t, pct) in rows.items():
+        assert t >= 0.0 and pct >= 0.0, (k, t, pct)
+    assert rows["Total"][1] <= 100.0 + 1e-6, rows["Total"]
+    assert abs(rows["Total"][1] + rows["The Rest"][1] - 100.0) < 1e-2
