@@ -58,6 +58,9 @@ MODULE tea_host
   LOGICAL :: complete = .FALSE., host_ready = .FALSE., out_is_file = .FALSE.
   INTEGER :: g_out = 6
   TYPE(profiler_type) :: profiler
+  ! visit output: written beside tea.out; the index file is started on the first dump
+  CHARACTER(LEN=LINE_LEN) :: out_dir = ''
+  LOGICAL :: visit_first = .TRUE.
   REAL(KIND=8) :: timerstart = 0.0_8, first_step_t = 0.0_8, second_step_t = 0.0_8
   ! SAVEd solver state of tea_solve.f90:55,58
   REAL(KIND=8) :: eigmin = 0.0_8, eigmax = 0.0_8, theta = 0.0_8, cn = 0.0_8
@@ -578,6 +581,68 @@ CONTAINS
       ENDIF
     ENDIF
   END SUBROUTINE field_summary
+
+  !> VTK dump of density, energy0 and u of the one chunk (visit.f90:23-125):
+  !> tea.visit lists the file of every dump; tea.00001.SSSSS.vtk holds the
+  !> rectilinear grid and the three cell fields, one value per line.
+  SUBROUTINE visit()
+    INTEGER :: u, j, ios
+    CHARACTER(LEN=10) :: chunk_name, step_name
+    CHARACTER(LEN=90) :: filename
+    REAL(KIND=8), ALLOCATABLE :: buf(:)
+    REAL(KIND=8) :: t0
+    IF (visit_first) THEN
+      OPEN(NEWUNIT=u, FILE=TRIM(out_dir) // 'tea.visit', STATUS='REPLACE', IOSTAT=ios)
+      IF (ios /= 0) CALL report_error('visit', 'Error opening tea.visit.')
+      WRITE(u, '(a,i5)') '!NBLOCKS ', 1
+      CLOSE(u)
+      visit_first = .FALSE.
+    ENDIF
+    WRITE(chunk_name, '(i6)') 1 + 100000
+    chunk_name(1:1) = '.'
+    WRITE(step_name, '(i6)') step + 100000
+    step_name(1:1) = '.'
+    filename = 'tea' // TRIM(chunk_name) // TRIM(step_name) // '.vtk'
+    OPEN(NEWUNIT=u, FILE=TRIM(out_dir) // 'tea.visit', STATUS='UNKNOWN', POSITION='APPEND', IOSTAT=ios)
+    IF (ios /= 0) CALL report_error('visit', 'Error opening tea.visit.')
+    WRITE(u, '(a)') TRIM(filename)
+    CLOSE(u)
+
+    IF (profiler_on) t0 = timer()
+    OPEN(NEWUNIT=u, FILE=TRIM(out_dir) // TRIM(filename), STATUS='REPLACE', IOSTAT=ios)
+    IF (ios /= 0) CALL report_error('visit', 'Error opening ' // TRIM(filename))
+    WRITE(u, '(a)') '# vtk DataFile Version 3.0'
+    WRITE(u, '(a)') 'vtk output'
+    WRITE(u, '(a)') 'ASCII'
+    WRITE(u, '(a)') 'DATASET RECTILINEAR_GRID'
+    WRITE(u, '(a,2i12,a)') 'DIMENSIONS', x_cells + 1, y_cells + 1, ' 1'
+    ! vertices as initialise_chunk_kernel.f90:61 computes them
+    WRITE(u, '(a,i5,a)') 'X_COORDINATES ', x_cells + 1, ' double'
+    DO j = 1, x_cells + 1
+      WRITE(u, '(e12.4)') xmin + dx * REAL(j - 1)
+    ENDDO
+    WRITE(u, '(a,i5,a)') 'Y_COORDINATES ', y_cells + 1, ' double'
+    DO j = 1, y_cells + 1
+      WRITE(u, '(e12.4)') ymin + dy * REAL(j - 1)
+    ENDDO
+    WRITE(u, '(a)') 'Z_COORDINATES 1 double'
+    WRITE(u, '(a)') '0'
+    WRITE(u, '(a,i20)') 'CELL_DATA ', x_cells * y_cells
+    WRITE(u, '(a)') 'FIELD FieldData 3'
+    ALLOCATE(buf(x_cells * y_cells))
+    CALL hip_check(tl_download(INT(FIELD_DENSITY, c_int), buf, 0_c_int), 'visit')
+    WRITE(u, '(a,i20,a)') 'density 1 ', x_cells * y_cells, ' double'
+    WRITE(u, '(e12.4)') buf
+    CALL hip_check(tl_download(INT(FIELD_ENERGY0, c_int), buf, 0_c_int), 'visit')
+    WRITE(u, '(a,i20,a)') 'energy 1 ', x_cells * y_cells, ' double'
+    WRITE(u, '(e12.4)') buf
+    CALL hip_check(tl_download(INT(FIELD_U, c_int), buf, 0_c_int), 'visit')
+    WRITE(u, '(a,i20,a)') 'temperature 1 ', x_cells * y_cells, ' double'
+    WRITE(u, '(e12.4)') buf
+    DEALLOCATE(buf)
+    CLOSE(u)
+    IF (profiler_on) profiler%visit = profiler%visit + (timer() - t0)
+  END SUBROUTINE visit
 
   SUBROUTINE timestep()
     REAL(KIND=8) :: t0
@@ -1107,6 +1172,8 @@ CONTAINS
     INTEGER :: ios, i
     INTEGER :: fields(NUM_FIELDS)
 
+    out_dir = out_path(1:INDEX(out_path, '/', BACK=.TRUE.))
+    visit_first = .TRUE.
     IF (LEN_TRIM(out_path) > 0) THEN
       OPEN(NEWUNIT=g_out, FILE=TRIM(out_path), ACTION='WRITE', STATUS='REPLACE', IOSTAT=ios)
       IF (ios /= 0) CALL report_error('initialise', 'Error opening tea.out file.')
@@ -1172,6 +1239,7 @@ CONTAINS
     WRITE(g_out, *) 'Problem initialised and generated'
     CALL hip_check(tl_set_field(), 'set_field')
     CALL field_summary()
+    IF (visit_frequency /= 0) CALL visit()
     WRITE(g_out, *) 'Starting the calculation'
     complete = .FALSE.
     ! the profile covers the timed run, so its total stays within the wall clock
@@ -1191,6 +1259,9 @@ CONTAINS
     IF (summary_frequency /= 0) THEN
       IF (MOD(step, summary_frequency) == 0) CALL field_summary()
     ENDIF
+    IF (visit_frequency /= 0) THEN
+      IF (MOD(step, visit_frequency) == 0) CALL visit()
+    ENDIF
     IF (step == 1) first_step_t = timer() - step_time
     IF (step == 2) second_step_t = timer() - step_time
     wall_clock = timer() - timerstart
@@ -1206,6 +1277,7 @@ CONTAINS
     IF (time + g_small > end_time .OR. step >= end_step) THEN
       complete = .TRUE.
       CALL field_summary()
+      IF (visit_frequency /= 0) CALL visit()
       wall_clock = timer() - timerstart
       WRITE(g_out, *)
       WRITE(g_out, *) 'Calculation complete'

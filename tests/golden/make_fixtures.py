@@ -4,7 +4,8 @@ Runs oracle/_ref/tea_leaf (the unmodified reference built from its sources by
 oracle/ref.mk) on one thread for every seeded deck below, and stores the
 parsed tea.out (per-step `step:` rows, iteration counts, switch/eigen lines)
 as JSON under tests/golden/fixtures/; for the layout_* decks also the line
-layout of the whole tea.out (teaout.skeleton).  Also converts the goldens shipped with
+layout of the whole tea.out (teaout.skeleton), for the visit_* decks the
+VTK files and tea.visit.  Also converts the goldens shipped with
 the reference (Benchmarks/tea_bm_N.{in,out}) into the same JSON form.
 
     python tests/golden/make_fixtures.py [name ...]
@@ -64,16 +65,26 @@ CASES = {
                            max_iters=500, inner_steps=4, end_step=2, profiler=True),
     "layout_jacobi_32": dict(cells=32, solver="jacobi", seed=14, eps=1e-6, max_iters=4000,
                              reflective=True, end_step=2, profiler=True),
+    # visit output: the VTK files and tea.visit are kept too
+    "visit_cg_16x12": dict(cells=16, y_cells=12, solver="cg", seed=15, end_step=3,
+                           extra=("visit_frequency=2",)),
 }
 
 
-def run_reference(deck):
+def run_reference(deck, others=None):
+    """tea.out of the reference on deck; with a dict `others`, also every other
+    file the run wrote (visit output), by name."""
     with tempfile.TemporaryDirectory() as d:
         with open(os.path.join(d, "tea.in"), "w") as f:
             f.write(deck)
         env = dict(os.environ, OMP_NUM_THREADS="1")
         subprocess.run([REF_BIN], cwd=d, env=env, check=True, stdout=subprocess.DEVNULL,
                        stderr=subprocess.DEVNULL, timeout=3600)
+        if others is not None:
+            for n in sorted(os.listdir(d)):
+                if n not in ("tea.in", "tea.out", "tea.in.tmp"):  # tea.in.tmp: the reader's copy
+                    with open(os.path.join(d, n)) as f:
+                        others[n] = f.read()
         with open(os.path.join(d, "tea.out")) as f:
             return f.read()
 
@@ -93,11 +104,14 @@ def main(names):
         cells = kw.pop("cells")
         deck = make_deck(cells, **kw)
         print("reference run:", name, flush=True)
-        text = run_reference(deck)
+        files = {}
+        text = run_reference(deck, files)
         fx = fixture_from(text, deck,
                           "oracle/_ref/tea_leaf (reference built from source), OMP_NUM_THREADS=1")
         if name.startswith("layout_"):
             fx["skeleton"] = teaout.skeleton(text)
+        if name.startswith("visit_"):
+            fx["files"] = files
         with open(os.path.join(OUT, name + ".json"), "w") as f:
             json.dump(fx, f, indent=1)
     # goldens shipped with the reference (data, converted)

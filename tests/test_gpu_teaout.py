@@ -74,3 +74,21 @@ def test_teaout_layout(name, tmp_path):
         assert t >= 0.0 and pct >= 0.0, (k, t, pct)
     assert rows["Total"][1] <= 100.0 + 1e-6, rows["Total"]
     assert abs(rows["Total"][1] + rows["The Rest"][1] - 100.0) < 1e-2
+
+
+def test_visit_files(tmp_path):
+    """visit_frequency: tea.visit and the VTK files (visit.f90:23-125) are the
+    reference's, character for character (four significant digits per value)."""
+    fx = FX.load("visit_cg_16x12")
+    print("[teaout] visit_cg_16x12", flush=True)
+    (tmp_path / "tea.in").write_text(fx["deck"])
+    subprocess.run([host.TEA_LEAF_BIN], cwd=tmp_path, check=True, timeout=120,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    written = sorted(p.name for p in tmp_path.iterdir()
+                     if p.name not in ("tea.in", "tea.out"))
+    assert written == sorted(fx["files"]), written
+    for name, want in fx["files"].items():
+        got = (tmp_path / name).read_text()
+        assert got.splitlines() == want.splitlines(), name
+    parsed = teaout.parse((tmp_path / "tea.out").read_text())
+    FX.compare_summaries(parsed["summaries"], fx["summaries"], 1e-10)
