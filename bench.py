@@ -246,12 +246,15 @@ def deck_fields(cells, coefficient="density"):
     return rho, e1, dt / (dx * dx), dt / (dx * dx)
 
 
-def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed=5):
+def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed=5,
+                  solver="cg"):
     """The adjoint case: gradient of J = sum over a seeded 256x256 window of
     (u_N - u_target)^2 with respect to per-cell log-density, u_target from a
-    forward on a log-normally perturbed density (sigma 0.1); timed forward and
-    backward (checkpoint every `every` steps); central differences with step
-    h on n_fd seeded cells inside the window."""
+    forward on a log-normally perturbed density (sigma 0.1), computed once
+    (by the first precision) and shared; timed forward and backward
+    (checkpoint every `every` steps) with CG or PPCG solves (PPCG: 10 inner
+    sweeps, bounds from 30 CG iterations of each solve); central differences
+    with step h on n_fd seeded cells inside the window."""
     import ctypes as C
     import numpy as np
     from tealeaf_ref_amd import _lib
@@ -264,19 +267,22 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
     win = (slice(wy, wy + 256), slice(wx, wx + 256))
     precs = ["fp64", "mixed"] if precision == "both" else [precision]
     out = {"workload": f"{cells}x{cells}, {steps} steps, seeded deck, per-cell density "
-                       f"rho*exp(0.1 xi), window 256x256 at ({wx},{wy}), checkpoint every {every}"}
+                       f"rho*exp(0.1 xi), window 256x256 at ({wx},{wy}), checkpoint every {every}, "
+                       f"{solver.upper()} solves"}
     grads = {}
+    target = None
     for prec in precs:
         p = TeaLeafSteps(e0, rx, ry, steps, coefficient=_lib.CONDUCTIVITY,
                          eps=1e-15 if prec == "fp64" else mixed_eps, checkpoint_every=every,
-                         precision=prec)
+                         precision=prec, solver=solver)
         if cells >= 4096:
             p.progress = lambda msg: log(f"[bench] adjoint {prec} {cells}^2: {msg}")
-        target = p.forward(theta_p)
+        if target is None:
+            target = p.forward(theta_p)
         t0 = time.perf_counter()
         u = p.forward(theta)
         t_f = time.perf_counter() - t0
-        its_f = p.iterations
+        its_f, sw_f = p.iterations, p.sweeps
         seed_g = np.zeros_like(u)
         seed_g[win] = 2.0 * (u[win] - target[win])
         J = float(((u[win] - target[win]) ** 2).sum())
@@ -290,6 +296,8 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
                "forward_backward_s_per_step": (t_f + t_b) / steps,
                "forward_iterations_per_step": its_f / steps,
                "backward_iterations_per_step": its_b / steps,
+               "forward_sweeps_per_step": sw_f / steps,
+               "backward_sweeps_per_step": p.sweeps / steps,
                "device_bytes": nbytes.value}
         fd_rng = np.random.default_rng(seed + 1)
         errs = []
@@ -406,6 +414,7 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=4, help="finite-difference cells")
     ap.add_argument("--adjoint-h", type=float, default=1e-6)
     ap.add_argument("--adjoint-precision", default="both", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--adjoint-solver", default="cg", choices=["cg", "ppcg"])
     ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
                     help="true relative residual of the mixed-precision solves")
     ap.add_argument("--no-small", action="store_true", help="skip the 512^2 CG case")
@@ -475,7 +484,7 @@ def main():
     if args.adjoint_cells:
         out["adjoint"] = adjoint_bench(args.adjoint_cells, args.adjoint_steps, args.adjoint_every,
                                        args.adjoint_fd, args.adjoint_h, args.adjoint_precision,
-                                       args.adjoint_mixed_eps)
+                                       args.adjoint_mixed_eps, solver=args.adjoint_solver)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap)

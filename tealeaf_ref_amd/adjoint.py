@@ -9,8 +9,10 @@ backward is the hand-written adjoint (tl_adjoint.hip): one solve with the
 same symmetric operator per step, plus the face-gradient kernel chained to
 log-density.  The forward keeps the energy every `checkpoint_every` steps in
 device snapshot slots; the backward recomputes each segment from its
-checkpoint and walks it in reverse.  Solves are FP64 CG, or the FP32-inner,
-FP64-refined solver (precision="mixed").
+checkpoint and walks it in reverse.  Solves are FP64 CG or PPCG, or the
+FP32-inner, FP64-refined CG or PPCG (precision="mixed"); solver="ppcg" takes
+its eigenvalue bounds from `presteps` CG iterations of every solve, so a
+recomputed step repeats the forward's arithmetic exactly.
 
 The library holds one chunk per process: one TeaLeafSteps at a time.
 """
@@ -42,13 +44,15 @@ class TeaLeafSteps:
 
     energy0: initial energy1 field (ny, nx); rx, ry: dt/dx^2, dt/dy^2;
     coefficient: _lib.CONDUCTIVITY or _lib.RECIP_CONDUCTIVITY; eps: relative
-    residual of every solve (recurrence residual for FP64 CG as in
-    tea_solve.f90:410, true residual for the mixed solver).
+    residual of every solve (recurrence residual for FP64 CG and PPCG as in
+    tea_solve.f90:410, true residual for the mixed solver); solver: "cg" or
+    "ppcg" (inner_steps Chebyshev sweeps per outer iteration, bounds from
+    presteps CG iterations).
     """
 
     def __init__(self, energy0, rx, ry, steps, coefficient=L.CONDUCTIVITY, eps=1e-15,
                  max_iters=20000, checkpoint_every=5, precision="fp64", reflective=False,
-                 device=0, mixed_inner_tol=1e-5):
+                 device=0, mixed_inner_tol=1e-5, solver="cg", inner_steps=10, presteps=30):
         e0 = np.asarray(energy0, dtype=np.float64)
         self.ny, self.nx = e0.shape
         self.rx, self.ry = float(rx), float(ry)
@@ -58,6 +62,13 @@ class TeaLeafSteps:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
+        if solver not in ("cg", "ppcg"):
+            raise ValueError("solver must be 'cg' or 'ppcg'")
+        self.solver = solver
+        self.inner_steps, self.presteps = int(inner_steps), int(presteps)
+        if solver == "ppcg" and (self.inner_steps < 1 or self.presteps < 2):
+            raise ValueError("ppcg needs inner_steps >= 1 and presteps >= 2")
+        self.sweeps = 0                # PPCG inner sweeps of the last forward/backward
         self.mixed_inner_tol = mixed_inner_tol
         self.reflective = bool(reflective)
         self.iterations = 0            # solver iterations of the last forward/backward
@@ -85,19 +96,69 @@ class TeaLeafSteps:
         if self.reflective:
             d.update_halo(_halo_flags("u"), 1)
         if self.precision == "mixed":
-            st = d.mixed_solve(solver=L.MIXED_CG, rel_tol=self.eps, inner_tol=self.mixed_inner_tol,
-                               max_inner=self.max_iters)
+            ppcg = self.solver == "ppcg"
+            st = d.mixed_solve(solver=L.MIXED_PPCG if ppcg else L.MIXED_CG, rel_tol=self.eps,
+                               inner_tol=self.mixed_inner_tol, max_inner=self.max_iters,
+                               inner_steps=self.inner_steps, presteps=self.presteps)
             self.iterations += st.inner_iters
+            self.sweeps += st.sweeps
             return
+        L.call("tl_set_ppcg_active", 0)
         L.call("tl_calc_residual")
         rr = d.scalar("tl_calc_2norm", 1)
         if rr == 0.0:
             return
         rro = d.scalar("tl_cg_init")
-        st = L.LoopState(n_first=1, n_last=self.max_iters, eps_abs=self.eps * math.sqrt(abs(rr)),
-                         check_switch=0, epslim=0.0, presteps=0, rro=rro, old_error=rr)
-        L.call("tl_cg_loop", C.byref(st), None, None)
-        self.iterations += st.n_end
+        eps_abs = self.eps * math.sqrt(abs(rr))
+        if self.solver == "cg":
+            st = L.LoopState(n_first=1, n_last=self.max_iters, eps_abs=eps_abs, check_switch=0,
+                             epslim=0.0, presteps=0, rro=rro, old_error=rr)
+            L.call("tl_cg_loop", C.byref(st), None, None)
+            self.iterations += st.n_end
+            return
+        self._solve_ppcg(rro, rr, eps_abs)
+
+    def _solve_ppcg(self, rro, rr, eps_abs):
+        """FP64 PPCG as tea_solve.f90:160-330 runs it: CG until the switch at
+        iteration `presteps`, eigenvalue bounds from the CG coefficients
+        (tea_leaf_cheby.f90:142-176, x0.95 / x1.05), Chebyshev coefficients for
+        the inner sweeps (tea_leaf_cheby.f90:178-205), then flexible CG with the
+        polynomial preconditioner to the exit test."""
+        d = self.dev
+        n_max = self.max_iters
+        al = np.zeros(n_max + 2)
+        be = np.zeros(n_max + 2)
+        pd = C.POINTER(C.c_double)
+        st = L.LoopState(n_first=1, n_last=n_max, eps_abs=eps_abs, check_switch=1,
+                         epslim=math.inf, presteps=self.presteps, rro=rro, old_error=rr)
+        L.call("tl_cg_loop", C.byref(st), al.ctypes.data_as(pd), be.ctypes.data_as(pd))
+        if st.reason != L.LOOP_SWITCH:
+            self.iterations += st.n_end
+            return
+        n = st.n_end
+        self.iterations += n - 1
+        emin, emax = lanczos_bounds(al[1:n], be[1:n])
+        ca, cb, theta = chebyshev_coefs(emin * 0.95, emax * 1.05, self.inner_steps)
+        L.call("tl_set_ppcg_active", 1)
+        d.scalar("tl_ppcg_init", 2)
+        L.call("tl_ppcg_init_sd", theta)
+        if self.reflective:
+            d.update_halo(_halo_flags("sd", "r"), 1)
+        L.call("tl_ppcg_inner", ca.ctypes.data_as(pd), cb.ctypes.data_as(pd), 1, self.inner_steps)
+        L.call("tl_ppcg_update_z")
+        rro = d.scalar("tl_ppcg_init", 3)
+        if self.reflective:
+            d.update_halo(_halo_flags("p"), 1)
+        sweeps = C.c_int(self.inner_steps)
+        st2 = L.LoopState(n_first=n, n_last=n_max, eps_abs=eps_abs, check_switch=0, epslim=0.0,
+                          presteps=0, rro=rro, old_error=st.old_error)
+        L.call("tl_ppcg_loop", C.byref(st2), ca.ctypes.data_as(pd), cb.ctypes.data_as(pd),
+               self.inner_steps, theta, C.byref(sweeps))
+        L.call("tl_set_ppcg_active", 0)
+        if st2.reason != L.LOOP_CONVERGED:
+            raise RuntimeError(f"PPCG did not converge in {n_max} iterations")
+        self.iterations += st2.n_end - n + 1
+        self.sweeps += sweeps.value
 
     def _step(self):
         d = self.dev
@@ -115,6 +176,7 @@ class TeaLeafSteps:
         self.dev.upload("density", _pad(np.exp(th)))
         L.call("tl_snapshot_load", self.E0, L.FIELD["energy1"])
         self.iterations = 0
+        self.sweeps = 0
         for n in range(1, self.steps + 1):
             if (n - 1) % self.k == 0:
                 L.call("tl_snapshot_save", self.CK + (n - 1) // self.k, L.FIELD["energy1"])
@@ -136,6 +198,7 @@ class TeaLeafSteps:
         L.call("tl_snapshot_upload", self.RHS, host.ctypes.data_as(C.POINTER(C.c_double)), 1)
         L.call("tl_snapshot_fill", self.GRAD, 0.0)
         self.iterations = 0
+        self.sweeps = 0
         nseg = (self.steps + self.k - 1) // self.k
         for seg in range(nseg - 1, -1, -1):
             a, b = seg * self.k, min(seg * self.k + self.k, self.steps)
@@ -162,6 +225,39 @@ class TeaLeafSteps:
     def apply(self, theta):
         """u_N as a differentiable function of a torch tensor theta (ny, nx)."""
         return _function()(theta, self)
+
+
+def lanczos_bounds(alphas, betas):
+    """Smallest and largest eigenvalue of the Lanczos tridiagonal of m CG
+    iterations with coefficients alpha_1..m, beta_1..m: diagonal
+    1/alpha_n + beta_{n-1}/alpha_{n-1}, off-diagonal sqrt(beta_n)/alpha_n
+    (tea_leaf_cheby.f90:156-160)."""
+    a = np.asarray(alphas, dtype=np.float64)
+    b = np.asarray(betas, dtype=np.float64)
+    diag = 1.0 / a
+    diag[1:] += b[:-1] / a[:-1]
+    off = np.sqrt(b[:-1]) / a[:-1]
+    t = np.diag(diag) + np.diag(off, 1) + np.diag(off, -1)
+    ev = np.linalg.eigvalsh(t)
+    if not (ev[0] > 0.0):
+        raise RuntimeError("eigenvalue estimate is not positive")
+    return float(ev[0]), float(ev[-1])
+
+
+def chebyshev_coefs(eigmin, eigmax, n):
+    """alpha_i, beta_i (i = 1..n) and theta of the Chebyshev recurrence
+    (tea_leaf_cheby.f90:178-205); arrays start at i = 1."""
+    theta = (eigmax + eigmin) / 2.0
+    delta = (eigmax - eigmin) / 2.0
+    sigma = theta / delta
+    rho_old = 1.0 / sigma
+    ca, cb = np.zeros(n), np.zeros(n)
+    for i in range(n):
+        rho_new = 1.0 / (2.0 * sigma - rho_old)
+        ca[i] = rho_new * rho_old
+        cb[i] = 2.0 * rho_new / delta
+        rho_old = rho_new
+    return ca, cb, theta
 
 
 _FN = None

@@ -130,3 +130,31 @@ def test_mixed_precision_adjoint(problems):
     rel = np.linalg.norm(grads["mixed"] - grads["fp64"]) / np.linalg.norm(grads["fp64"])
     print(f"  mixed vs FP64 gradient rel L2 {rel:.2e}", flush=True)
     assert rel <= 1e-7
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_ppcg_adjoint(problems, prec):
+    """solver="ppcg": the forward and the gradient agree with FP64 CG's (the
+    same linear systems solved to a tight tolerance), PPCG really ran (inner
+    sweeps counted) and checkpointing stays bit-identical."""
+    theta, e0, target, win = case(19)
+    ref = problems(e0, 20.0, 20.0, 4, checkpoint_every=2, eps=1e-15)
+    u_ref = ref.forward(theta)
+    g_ref = ref.backward(seed_grad(u_ref, target, win))
+    ref.close()
+    grads = []
+    for k in (1, 4):
+        p = problems(e0, 20.0, 20.0, 4, checkpoint_every=k, precision=prec, solver="ppcg",
+                     inner_steps=6, presteps=10, eps=1e-15 if prec == "fp64" else 1e-12)
+        print(f"ppcg adjoint {prec} checkpoint {k}", flush=True)
+        u = p.forward(theta)
+        assert p.sweeps > 0
+        du = np.max(np.abs(u - u_ref)) / np.max(np.abs(u_ref))
+        grads.append(p.backward(seed_grad(u, target, win)))
+        assert p.sweeps > 0
+        p.close()
+        rel = np.linalg.norm(grads[-1] - g_ref) / np.linalg.norm(g_ref)
+        print(f"  u max rel {du:.2e}, gradient rel L2 vs FP64 CG {rel:.2e}", flush=True)
+        assert du <= (1e-12 if prec == "fp64" else 1e-9)
+        assert rel <= (1e-10 if prec == "fp64" else 1e-7)
+    assert np.array_equal(grads[0], grads[1])
