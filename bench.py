@@ -247,14 +247,20 @@ def deck_fields(cells, coefficient="density"):
 
 
 def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed=5,
-                  solver="cg"):
+                  solver="cg", save=None, compare=None):
     """The adjoint case: gradient of J = sum over a seeded 256x256 window of
     (u_N - u_target)^2 with respect to per-cell log-density, u_target from a
     forward on a log-normally perturbed density (sigma 0.1), computed once
     (by the first precision) and shared; timed forward and backward
     (checkpoint every `every` steps) with CG or PPCG solves (PPCG: 10 inner
     sweeps, bounds from 30 CG iterations of each solve); central differences
-    with step h on n_fd seeded cells inside the window."""
+    with step h on n_fd seeded cells inside the window.
+
+    A case too long for one run is split in two: `save` writes the target on
+    the window, the gradient on the window and on every 8th row and column,
+    and its norm (.npz); a later run of the other precision with `compare`
+    takes the target from that file and reports its gradient's difference
+    from the saved one (relative L2 on the window and on the subsample)."""
     import ctypes as C
     import numpy as np
     from tealeaf_ref_amd import _lib
@@ -271,10 +277,22 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
                        f"{solver.upper()} solves"}
     grads = {}
     target = None
+    saved = None
+    if compare:
+        saved = dict(np.load(compare))
+        target = np.zeros((cells, cells))
+        target[win] = saved["target_window"]
+        out["compared_with"] = {"file": os.path.basename(compare), "J": float(saved["J"]),
+                                "precision": str(saved["precision"]),
+                                "solver": str(saved["solver"])}
     for prec in precs:
+        # FP64: the reference's 1e-15 for CG; PPCG's flexible-CG residual
+        # (r - r_store).z stalls at the rounding floor near there, so 1e-14
+        fp64_eps = 1e-15 if solver == "cg" else 1e-14
         p = TeaLeafSteps(e0, rx, ry, steps, coefficient=_lib.CONDUCTIVITY,
-                         eps=1e-15 if prec == "fp64" else mixed_eps, checkpoint_every=every,
-                         precision=prec, solver=solver)
+                         eps=fp64_eps if prec == "fp64" else mixed_eps, checkpoint_every=every,
+                         precision=prec, solver=solver,
+                         max_iters=20000 if solver == "cg" else 5000)
         if cells >= 4096:
             p.progress = lambda msg: log(f"[bench] adjoint {prec} {cells}^2: {msg}")
         if target is None:
@@ -282,7 +300,7 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
         t0 = time.perf_counter()
         u = p.forward(theta)
         t_f = time.perf_counter() - t0
-        its_f, sw_f = p.iterations, p.sweeps
+        its_f, sw_f, unc_f = p.iterations, p.sweeps, p.unconverged
         seed_g = np.zeros_like(u)
         seed_g[win] = 2.0 * (u[win] - target[win])
         J = float(((u[win] - target[win]) ** 2).sum())
@@ -298,6 +316,7 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
                "backward_iterations_per_step": its_b / steps,
                "forward_sweeps_per_step": sw_f / steps,
                "backward_sweeps_per_step": p.sweeps / steps,
+               "eps": p.eps, "unconverged_solves": unc_f + p.unconverged,
                "device_bytes": nbytes.value}
         fd_rng = np.random.default_rng(seed + 1)
         errs = []
@@ -315,6 +334,17 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
         if errs:
             rec.update({"fd_cells": n_fd, "fd_h": h, "fd_max_rel_err": max(errs),
                         "fd_median_rel_err": float(np.median(errs))})
+        if save:
+            np.savez(save, target_window=target[win], g_window=g[win], g_sub=g[::8, ::8],
+                     g_norm=np.linalg.norm(g), J=J, precision=prec, solver=solver)
+        if saved is not None:
+            gw, gs = saved["g_window"], saved["g_sub"]
+            rec["vs_saved_gradient_rel_l2_window"] = float(
+                np.linalg.norm(g[win] - gw) / np.linalg.norm(gw))
+            rec["vs_saved_gradient_rel_l2_subsample"] = float(
+                np.linalg.norm(g[::8, ::8] - gs) / np.linalg.norm(gs))
+            rec["vs_saved_gradient_norm_rel"] = float(
+                abs(np.linalg.norm(g) - saved["g_norm"]) / saved["g_norm"])
         log(f"[bench] adjoint {cells}^2 {prec}: {rec}")
         out[prec] = rec
         grads[prec] = g
@@ -415,6 +445,11 @@ def main():
     ap.add_argument("--adjoint-h", type=float, default=1e-6)
     ap.add_argument("--adjoint-precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--adjoint-solver", default="cg", choices=["cg", "ppcg"])
+    ap.add_argument("--adjoint-save", default=None,
+                    help="write target, gradient window/subsample and norm to this .npz")
+    ap.add_argument("--adjoint-compare", default=None,
+                    help="take the target from this .npz (--adjoint-save of an earlier run) "
+                         "and compare the gradient with the one saved there")
     ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
                     help="true relative residual of the mixed-precision solves")
     ap.add_argument("--no-small", action="store_true", help="skip the 512^2 CG case")
@@ -484,7 +519,8 @@ def main():
     if args.adjoint_cells:
         out["adjoint"] = adjoint_bench(args.adjoint_cells, args.adjoint_steps, args.adjoint_every,
                                        args.adjoint_fd, args.adjoint_h, args.adjoint_precision,
-                                       args.adjoint_mixed_eps, solver=args.adjoint_solver)
+                                       args.adjoint_mixed_eps, solver=args.adjoint_solver,
+                                       save=args.adjoint_save, compare=args.adjoint_compare)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap)

@@ -235,8 +235,10 @@ int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int step_first
  * reference) whose dot products, alpha/beta and eigenvalue bounds are FP64,
  * and adds u += d in FP64.  It stops when ||u0 - A u|| <= rel_tol times the
  * residual of the starting u (a true residual, unlike the recurrence
- * residual of tea_solve.f90:402-410).  Requires tl_preconditioner_type none
- * and a chunk with all four faces external. */
+ * residual of tea_solve.f90:402-410), after max_outer refinements, or when
+ * two refinements in a row fail to halve the true residual (it has reached
+ * its rounding floor; reason TL_LOOP_MAXITERS).  Requires
+ * tl_preconditioner_type none and a chunk with all four faces external. */
 #define TL_MIXED_CG 1
 #define TL_MIXED_PPCG 2
 

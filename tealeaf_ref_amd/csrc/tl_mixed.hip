@@ -359,7 +359,8 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
   int pcur = 0, sdcur = 0;
   LoopBufs<float> b = f32_bufs(pcur, sdcur);
   double res = r0;
-  while (res > target && st->outer < st->max_outer) {
+  int stalls = 0;  // refinements in a row that did not halve the true residual
+  while (res > target && st->outer < st->max_outer && stalls < 2) {
     if ((rc = zero_d())) return rc;
     // ask an inner solve for no more than the remaining reduction (with a
     // margin), and never less than inner_tol
@@ -415,8 +416,12 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
       st->sweeps += sweeps;
     }
     if ((rc = refine(true, &rr64, &rr32))) return rc;
+    const double prev = res;
     res = sqrt(rr64);
     st->outer += 1;
+    // the true residual has reached its rounding floor (a target below it,
+    // e.g. relative to a small initial residual, cannot be met)
+    stalls = res > 0.5 * prev ? stalls + 1 : 0;
     st->final_residual = res;
   }
   st->reason = res <= target ? TL_LOOP_CONVERGED : TL_LOOP_MAXITERS;
