@@ -6,6 +6,7 @@
 #include <cstdlib>
 
 #include "tl_loopk.hpp"
+#include "tl_cheby2.hpp"
 
 namespace tl {
 dev::Geo geo(int cols_per_tile);
@@ -171,6 +172,85 @@ __global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_v(Geo G, const 
   }
 }
 
+// Chebyshev iterations k and k+1 in one pass (tl_cheby2.hpp): u and p are
+// read from uin / pin and written to uout / pout.  On the last pass of a batch
+// w, r (and z) of iteration k+1 are stored and the partials of r.r written.
+// compiled for 4 waves per SIMD, so that a 4096^2 grid (1024 blocks of 4
+// waves) is resident at once; this spills 29 registers, and 3 waves (no
+// spills, a quarter of the blocks in a second round) measured the same
+#ifndef TL_CHEBY_PAIR_WAVES
+#define TL_CHEBY_PAIR_WAVES 4
+#endif
+
+template <bool NT>
+__global__ __launch_bounds__(BLOCK, TL_CHEBY_PAIR_WAVES) void k_cheby_pair(
+    Geo G, const double *__restrict__ uin, double *__restrict__ uout,
+    const double *__restrict__ u0, const double *__restrict__ Kx, const double *__restrict__ Ky,
+    const double *__restrict__ pin, double *__restrict__ pout, double *__restrict__ w,
+    double *__restrict__ r, double *__restrict__ z, const double *__restrict__ Mi, double rx,
+    double ry, double a1, double b1, double a2, double b2, int diag, int last,
+    double *__restrict__ out_part, int dir) {
+  typedef typename dev::Vec<double, 2>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<2> tt = dev::tilev_of<2>(G, t);
+    auto ldop = [&](int col, int row) -> VT {
+      return dev::ldv<NT, double, 2>(uin + (long)row * P + col);
+    };
+    auto ld1 = [&](int col, int row) -> double { return uin[(long)row * P + col]; };
+    auto side = [&](int col, int row) -> dev::ChebyIn<VT> {
+      const long i = (long)row * P + col;
+      dev::ChebyIn<VT> e;
+      e.u0 = dev::ldv<NT, double, 2>(u0 + i);
+      e.p = dev::ldv<NT, double, 2>(pin + i);
+      e.mi = diag ? dev::ldv<NT, double, 2>(Mi + i) : VT(0);
+      return e;
+    };
+    auto side1 = [&](int col, int row) -> dev::ChebyIn<double> {
+      const long i = (long)row * P + col;
+      dev::ChebyIn<double> e;
+      e.u0 = u0[i];
+      e.p = pin[i];
+      e.mi = diag ? Mi[i] : 0.0;
+      return e;
+    };
+    dev::walk_cheby_pair<NT, 2>(
+        G, tt, Kx, Ky, diag, rx, ry, a1, b1, ldop, ld1, side, side1,
+        [&](int y, VT u1, VT p1, VT wv, const dev::ChebyIn<VT> &e) {
+      if (tt.c >= G.nx) return;
+      const long row = (long)y * P;
+      const VT rv = e.u0 - wv;
+      const VT src = diag ? e.mi * rv : rv;
+      const VT pv = a2 * p1 + b2 * src;
+      dev::stv<false, double, 2>(pout + row, tt.c, G.nx, pv);
+      dev::stv<false, double, 2>(uout + row, tt.c, G.nx, u1 + pv);
+      if (last) {
+        dev::stv<false, double, 2>(w + row, tt.c, G.nx, wv);
+        dev::stv<false, double, 2>(r + row, tt.c, G.nx, rv);
+        if (diag) dev::stv<false, double, 2>(z + row, tt.c, G.nx, src);
+        acc += rv.x * rv.x;
+        if (tt.c + 1 < G.nx) acc += rv.y * rv.y;
+      }
+      // carry the halo ring of u to the new buffer
+      if (tt.c == 0) uout[row - 1] = uin[row - 1];
+      if (tt.c + 2 >= G.nx) uout[row + G.nx] = uin[row + G.nx];
+      for (int side_row = 0; side_row < 2; ++side_row) {
+        if (y != (side_row ? G.ny - 1 : 0)) continue;
+        const long hr = side_row ? (long)G.ny * P : -P;
+        uout[hr + tt.c] = uin[hr + tt.c];
+        if (tt.c + 1 < G.nx) uout[hr + tt.c + 1] = uin[hr + tt.c + 1];
+      }
+    }, dir);
+  }
+  if (last) {
+    acc = dev::block_sum(acc, lds);
+    if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+  }
+}
+
 }  // namespace
 
 namespace tl {
@@ -186,19 +266,34 @@ int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_fi
   const int diag = s.precond == TL_PREC_JAC_DIAG;
   const bool nt = use_nontemporal();
   auto k = nt ? k_cheby_v<true> : k_cheby_v<false>;
-  for (int i = 0; i < count; ++i) {
+  auto k2 = nt ? k_cheby_pair<true> : k_cheby_pair<false>;
+  const bool pairs = sweep_pairs();
+  for (int i = 0; i < count;) {
     const int st = step_first + i - 1;
-    const int last = i == count - 1;
     const int in = s.ucur, outb = 1 - s.ucur;
     hipEvent_t ev;
     prof_begin(3, &ev);
-    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
-                       s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
-                       s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], diag, last, part(P0),
-                       s.walk_dir);
-    s.walk_dir = -s.walk_dir;  // the next iteration starts on the rows written last
+    if (pairs && i + 1 < count) {
+      // two iterations per pass; p moves to the other buffer
+      const int last = i + 2 == count;
+      hipLaunchKernelGGL(k2, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
+                         s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.p[1 - s.pcur].p11,
+                         s.w.p11, s.r.p11, s.z.p11, s.Mi.p11, s.rx, s.ry, ch_alphas[st],
+                         ch_betas[st], ch_alphas[st + 1], ch_betas[st + 1], diag, last, part(P0),
+                         s.walk_dir);
+      s.pcur = 1 - s.pcur;
+      i += 2;
+    } else {
+      const int last = i + 1 == count;
+      hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
+                         s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11,
+                         s.z.p11, s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], diag, last,
+                         part(P0), s.walk_dir);
+      i += 1;
+    }
+    s.walk_dir = -s.walk_dir;  // the next pass starts on the rows written last
     prof_end(3, ev);
-    if (hipGetLastError() != hipSuccess) return -fail(TL_ERR_HIP, "k_cheby_v launch");
+    if (hipGetLastError() != hipSuccess) return -fail(TL_ERR_HIP, "Chebyshev pass launch");
     s.ucur = outb;
   }
   if (want_norm && count > 0) {

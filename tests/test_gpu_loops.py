@@ -208,6 +208,47 @@ def test_cheby_run(pair_factory):
     p.cmp("u", tol=1e-10)
 
 
+@pytest.mark.parametrize("precond,count,nx,ny", [(1, 20, 1100, 70), (2, 21, 1100, 70),
+                                                (1, 7, 1100, 1400), (1, 9, 37, 5)])
+def test_cheby_pairs_bit_identical(precond, count, nx, ny, monkeypatch):
+    """Two Chebyshev iterations per pass (temporal blocking with ghost rows and
+    columns) compute what two single passes compute; an odd count ends with a
+    single pass.  The same operations on the same operands, so a difference is
+    the compiler's choice of fused multiply-adds (last-bit level); a wrong
+    ghost cell would show as O(1).  The first case is also checked against the
+    oracle."""
+    results = []
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("TL_SWEEP_PAIRS", pairs)
+        print(f"cheby pairs={pairs} precond={precond} count={count} {nx}x{ny}", flush=True)
+        p = Pair(nx, ny, 1, seed=43, precond=precond)
+        try:
+            _setup(p)
+            theta = 20.0
+            ch_a, ch_b, _ = O.calc_ch_coefs(1.0, 39.0, count + 2)
+            L.call("tl_cheby_init", theta)
+            a = np.ascontiguousarray(ch_a)
+            b = np.ascontiguousarray(ch_b)
+            rr = C.c_double()
+            L.call("tl_cheby_run", a.ctypes.data_as(C.POINTER(C.c_double)),
+                   b.ctypes.data_as(C.POINTER(C.c_double)), 1, count, 1, C.byref(rr))
+            if pairs == "1" and nx == 1100 and ny == 70 and precond == 1:
+                p.o.cheby_init(theta, precond)
+                for s_ in range(1, count + 1):
+                    p.o.cheby_iterate(ch_a[s_ - 1], ch_b[s_ - 1], precond)
+                assert_scalar(rr.value, p.o.calc_2norm(1), 1e-8, "cheby residual")
+                p.cmp("u", tol=1e-10)
+            results.append((rr.value, [p.d.download(f) for f in ("u", "p", "w", "r")]))
+        finally:
+            p.close()
+    (r0, f0), (r1, f1) = results
+    assert abs(r0 - r1) <= 1e-12 * abs(r0), (r0, r1)
+    for name, x, y in zip(("u", "p", "w", "r"), f0, f1):
+        d = np.max(np.abs(x - y)) / np.max(np.abs(x))
+        print(f"  pairs vs single passes: {name} max rel diff {d:.1e}", flush=True)
+        assert d <= 1e-12, (name, d)
+
+
 @pytest.mark.parametrize("precond,reflective,inner,ny", [(1, False, 6, 70), (2, False, 7, 70),
                                                           (1, True, 5, 70), (1, False, 4, 1400)])
 def test_ppcg_sweep_pairs_bit_identical(precond, reflective, inner, ny, monkeypatch):
