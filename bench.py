@@ -29,9 +29,40 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HIP events bracket every EVENT_PERIOD-th launch of each kernel class in the
+# timed steps: events around every launch slow a CG step by ~6 %
+EVENT_PERIOD = 16
 CG_W_BYTES_PER_CELL = 48.0      # fused calc_w: r, p, Kx, Ky in; p, w out
 CG_UR_BYTES_PER_CELL = 48.0     # calc_ur: u, p, r, w in; u, r out
 CG_ITER_BYTES_PER_CELL = 96.0
+# algorithmic bytes per cell at the fusion level the reference's dataflow
+# permits (each pass reads each input once and writes each output once, no
+# reuse across iterations): Chebyshev iteration, PPCG outer iteration without
+# its sweeps, PPCG sweep, and the fixed work of a step (init, residual,
+# cg_init, finalise, summary)
+CHEBY_ITER_BYTES_PER_CELL = 56.0
+PPCG_OUTER_BYTES_PER_CELL = 112.0
+PPCG_SWEEP_BYTES_PER_CELL = 64.0
+STEP_FIXED_BYTES_PER_CELL = 200.0
+
+
+def model_bytes_per_cell(solver, rows):
+    """Algorithmic bytes per cell of the steps in `rows` (TeaHost.last()):
+    the CG iterations before the switch, then Chebyshev iterations or PPCG
+    outer iterations and sweeps.  Temporal blocking (two sweeps or iterations
+    per pass) moves fewer bytes than this, so the resulting fraction of the
+    roofline can exceed 1."""
+    total = 0.0
+    for r in rows:
+        its = r["iterations"]
+        cg = its if solver == "cg" or r["switch_n"] <= 0 else r["switch_n"] - 1
+        rest = its - cg
+        total += STEP_FIXED_BYTES_PER_CELL + CG_ITER_BYTES_PER_CELL * cg
+        if solver == "chebyshev":
+            total += CHEBY_ITER_BYTES_PER_CELL * rest
+        elif solver == "ppcg":
+            total += PPCG_OUTER_BYTES_PER_CELL * rest + PPCG_SWEEP_BYTES_PER_CELL * r["ppcg_inner"]
+    return total
 
 
 def measured_traffic(kernel, cells):
@@ -70,7 +101,7 @@ def gpu_run(cells, solver, steps, warmup, workdir, **deck_kw):
         log(f"[bench] warmup step {i + 1}: {h.last()['iterations']:.0f} iterations")
     _lib.call("tl_synchronize")
     _lib.call("tl_profile_reset")
-    _lib.call("tl_profile_enable", 1)
+    _lib.call("tl_profile_enable", EVENT_PERIOD)
     iters = []
     rows = []
     t0 = time.perf_counter()
@@ -113,11 +144,15 @@ def solver_runs(cells, steps, warmup):
         rec = {"s_per_step": elapsed / steps, "iterations_per_step": sum(iters) / steps}
         if solver == "ppcg":
             rec["sweeps_per_step"] = sum(r["ppcg_inner"] for r in prof["rows"]) / steps
+        model = model_bytes_per_cell(solver, prof["rows"])
+        rec["model_bytes_per_cell_step"] = model / steps
+        rec["model_gbs"] = model * cells * cells / elapsed / 1e9
+        rec["model_frac"] = rec["model_gbs"] / PEAK_HBM_GBS
         kern = {}
         for cls, (name, bpc) in CLASS_BYTES.items():
             n, ms = prof[cls]
             if n:
-                kern[name] = {"launches": n, "avg_launch_ms": ms / n,
+                kern[name] = {"timed_launches": n, "avg_launch_ms": ms / n,
                               "achieved_gbs": bpc * cells * cells * n / (ms * 1e-3) / 1e9,
                               "bytes_per_cell": bpc}
         rec["kernels"] = kern
@@ -477,9 +512,11 @@ def main():
     mean_iters = sum(iters) / len(iters)
     n_w, ms_w = prof[0]
     n_ur, ms_ur = prof[1]
-    # achieved = algorithmic bytes of the executed launches / their event time
-    w_gbs = CG_W_BYTES_PER_CELL * cells * exec_iters / (ms_w * 1e-3) / 1e9 if ms_w else None
-    ur_gbs = CG_UR_BYTES_PER_CELL * cells * exec_iters / (ms_ur * 1e-3) / 1e9 if ms_ur else None
+    # achieved = algorithmic bytes per launch / the average event-timed launch
+    # (every EVENT_PERIOD-th launch; like rocprof's average, it includes the
+    # few launches queued past the exit that return at once)
+    w_gbs = CG_W_BYTES_PER_CELL * cells / (ms_w / n_w * 1e-3) / 1e9 if ms_w else None
+    ur_gbs = CG_UR_BYTES_PER_CELL * cells / (ms_ur / n_ur * 1e-3) / 1e9 if ms_ur else None
     step_gbs = CG_ITER_BYTES_PER_CELL * cells * exec_iters / elapsed / 1e9
     out = {
         "metric": "s/timestep + HBM GB/s at 4096^2: FP64 CG vs FP32-refined PPCG @1e-10 residual",
@@ -499,6 +536,7 @@ def main():
                    "cells": cells, "solver": args.solver, "eps": 1e-15,
                    "mean_iterations_per_step": mean_iters},
         "hbm_gbs_step": step_gbs,
+        "model_gbs_step": model_bytes_per_cell("cg", prof["rows"]) * cells / elapsed / 1e9,
         "cell_iterations_per_s": cells * exec_iters / elapsed,
         "roofline": {"bound": "hbm", "kernel": "cg calc_w (w = A p, p = r + beta p, p.w)",
                      "achieved": w_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -506,7 +544,8 @@ def main():
                      "traffic": measured_traffic("k_loop_w", args.cells),
                      "traffic_source": "profiles/cg_fp64_4096_hbm_traffic.md",
                      "bytes_per_cell": CG_W_BYTES_PER_CELL,
-                     "avg_launch_ms": ms_w / n_w if n_w else None, "launches": n_w,
+                     "avg_launch_ms": ms_w / n_w if n_w else None, "timed_launches": n_w,
+                     "event_sample_period": EVENT_PERIOD,
                      "calc_ur_achieved": ur_gbs,
                      "calc_ur_avg_launch_ms": ms_ur / n_ur if n_ur else None},
     }

@@ -300,9 +300,11 @@ int tl_adjoint_face_grad(int lambda_field, int u_slot, int e0_slot, int grad_slo
 int tl_device_bytes(double *bytes);
 
 /* ---- timing of the dominant kernels (HIP events on the library stream) --- */
-/* Enables per-launch HIP-event timing of the solver kernels. */
+/* on = 1 times every launch of the solver kernels with HIP events, on = N > 1
+ * every N-th launch of each class (events between launches slow a loop by a
+ * few per cent; sampling keeps that out of a timed run), on = 0 stops. */
 int tl_profile_enable(int on);
-/* Returns launch count and total milliseconds for a kernel class:
+/* Returns timed launch count and their total milliseconds for a kernel class:
  * 0 = CG calc_w, 1 = CG calc_ur, 2 = PPCG inner sweep, 3 = Chebyshev iterate,
  * 4 = Jacobi, 5 = PPCG calc_w, 6 = PPCG calc_ur, 7 = mixed-precision FP64
  * refinement (u += d, true residual). */

@@ -84,6 +84,8 @@ struct State {
 
   // profiling
   bool prof = false;
+  int prof_period = 1;      // time every prof_period-th launch of each class
+  long prof_seq[8] = {0};   // launches seen per class while enabled
   struct Ev { hipEvent_t a, b; int cls; };
   std::vector<Ev> ev_pending;
   std::vector<hipEvent_t> ev_pool;

@@ -45,6 +45,7 @@ void prof_begin(int cls, hipEvent_t *a) {
   State &s = S();
   *a = nullptr;
   if (!s.prof) return;
+  if (cls >= 0 && cls < 8 && (s.prof_seq[cls]++ % s.prof_period) != 0) return;
   if (s.ev_pool.empty()) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -53,7 +54,6 @@ void prof_begin(int cls, hipEvent_t *a) {
   *a = s.ev_pool.back();
   s.ev_pool.pop_back();
   (void)hipEventRecord(*a, s.stream);
-  (void)cls;
 }
 
 void prof_end(int cls, hipEvent_t a) {
@@ -310,7 +310,9 @@ extern "C" int tl_download(int field_id, double *host, int host_halo) {
 // --------------------------------------------------------------------------
 extern "C" int tl_profile_enable(int on) {
   TL_REQUIRE_INIT();
+  if (on < 0) return fail(TL_ERR_ARG, "tl_profile_enable: negative period");
   S().prof = on != 0;
+  S().prof_period = on > 1 ? on : 1;
   return TL_OK;
 }
 
@@ -343,6 +345,7 @@ extern "C" int tl_profile_reset(void) {
   for (int i = 0; i < 8; ++i) {
     S().prof_count[i] = 0;
     S().prof_ms[i] = 0.0;
+    S().prof_seq[i] = 0;
   }
   return TL_OK;
 }

@@ -286,3 +286,31 @@ def test_ppcg_sweep_pairs_bit_identical(precond, reflective, inner, ny, monkeypa
         d = np.max(np.abs(a - b)) / np.max(np.abs(a))
         print(f"  pairs vs single sweeps: {name} max rel diff {d:.1e}", flush=True)
         assert d <= 1e-12, (name, d)
+
+
+def test_profile_sampling():
+    """tl_profile_enable(N) times every N-th launch of a kernel class: the
+    same CG loop queues the same launches, of which ceil(n / N) are timed."""
+    counts = []
+    for period in (1, 4):
+        print(f"profile sampling period {period}", flush=True)
+        p = Pair(200, 150, 1, seed=47)
+        try:
+            _setup(p)
+            r0 = p.d.scalar("tl_calc_2norm", 1)
+            rro = p.d.scalar("tl_cg_init")
+            L.call("tl_profile_reset")
+            L.call("tl_profile_enable", period)
+            st = L.LoopState(n_first=1, n_last=300, eps_abs=1e-10 * math.sqrt(r0), rro=rro,
+                             old_error=r0)
+            L.call("tl_cg_loop", C.byref(st), None, None)
+            L.call("tl_profile_enable", 0)
+            n, ms = C.c_long(), C.c_double()
+            L.call("tl_profile_get", 0, C.byref(n), C.byref(ms))
+            counts.append((n.value, ms.value, st.n_end))
+        finally:
+            p.close()
+    (n1, ms1, e1), (n4, ms4, e4) = counts
+    assert e1 == e4 and n1 > 8
+    assert n4 == (n1 + 3) // 4, (n1, n4)
+    assert ms1 > 0.0 and ms4 > 0.0
