@@ -10,6 +10,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+# the same source built with multiply-add contraction (a second rounding)
+_LIB_FMA = os.path.join(_HERE, "liboracle_fma.so")
 
 PREC = {"none": 1, "jac_diag": 2, "jac_block": 3}
 FIELD_IDS = {"density": 1, "energy0": 2, "energy1": 3, "u": 4, "p": 5, "sd": 6,
@@ -57,59 +59,70 @@ class StepRecord(C.Structure):
 
 
 _lib = None
+_lib_fma = None
 
 
-def lib():
-    global _lib
+def lib(contracted=False):
+    """The plain build, or (contracted=True) the multiply-add build."""
+    global _lib, _lib_fma
+    if contracted:
+        if _lib_fma is None:
+            if not os.path.exists(_LIB_FMA):
+                raise RuntimeError("oracle/liboracle_fma.so is not built (run make -C oracle)")
+            _lib_fma = _configure(C.CDLL(_LIB_FMA))
+        return _lib_fma
     if _lib is None:
         if not os.path.exists(_LIB):
             raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
-        L = C.CDLL(_LIB)
-        P, D, I = C.POINTER, C.c_double, C.c_int
-        PC = P(Chunk)
-        L.tlo_parse_deck.argtypes = [C.c_char_p, P(Params), C.c_char_p, I]
-        L.tlo_run.argtypes = [P(Params), P(StepRecord), I, I, C.c_char_p, I]
-        L.tlo_chunk_create.argtypes = [I, I, I]
-        L.tlo_chunk_create.restype = PC
-        L.tlo_chunk_destroy.argtypes = [PC]
-        L.tlo_field.argtypes = [PC, I]
-        L.tlo_field.restype = P(D)
-        L.tlo_initialise_chunk.argtypes = [PC, P(Params)]
-        L.tlo_generate_chunk.argtypes = [PC, P(Params)]
-        L.tlo_set_field.argtypes = [PC]
-        L.tlo_common_init.argtypes = [PC, P(I), I, I, I]
-        for name in ("tlo_finalise", "tlo_calc_residual", "tlo_diag_init", "tlo_block_init",
-                     "tlo_ppcg_update_z", "tlo_ppcg_store_r"):
-            getattr(L, name).argtypes = [PC]
-        for name in ("tlo_calc_2norm",):
-            getattr(L, name).argtypes = [PC, I]
-            getattr(L, name).restype = D
-        L.tlo_cg_init.argtypes = [PC, I]
-        L.tlo_cg_init.restype = D
-        L.tlo_cg_calc_w.argtypes = [PC]
-        L.tlo_cg_calc_w.restype = D
-        L.tlo_cg_calc_ur.argtypes = [PC, D, I]
-        L.tlo_cg_calc_ur.restype = D
-        L.tlo_cg_calc_p.argtypes = [PC, D, I, I]
-        L.tlo_cheby_init.argtypes = [PC, D, I]
-        L.tlo_cheby_iterate.argtypes = [PC, D, D, I]
-        L.tlo_ppcg_init_sd.argtypes = [PC, D, I]
-        L.tlo_ppcg_init.argtypes = [PC, I, I, I]
-        L.tlo_ppcg_init.restype = D
-        L.tlo_ppcg_inner.argtypes = [PC, D, D, I]
-        L.tlo_ppcg_calc_zrnorm.argtypes = [PC, I, I]
-        L.tlo_ppcg_calc_zrnorm.restype = D
-        L.tlo_ppcg_calc_rrn.argtypes = [PC]
-        L.tlo_ppcg_calc_rrn.restype = D
-        L.tlo_jacobi_solve.argtypes = [PC]
-        L.tlo_jacobi_solve.restype = D
-        L.tlo_update_halo.argtypes = [PC, P(I), I, I]
-        L.tlo_field_summary.argtypes = [PC, P(D)]
-        L.tlo_tqli.argtypes = [P(D), P(D), I]
-        L.tlo_calc_eigenvalues.argtypes = [P(D), P(D), I, P(D), P(D)]
-        L.tlo_calc_ch_coefs.argtypes = [P(D), P(D), D, D, P(D), I]
-        _lib = L
+        _lib = _configure(C.CDLL(_LIB))
     return _lib
+
+
+def _configure(L):
+    P, D, I = C.POINTER, C.c_double, C.c_int
+    PC = P(Chunk)
+    L.tlo_parse_deck.argtypes = [C.c_char_p, P(Params), C.c_char_p, I]
+    L.tlo_run.argtypes = [P(Params), P(StepRecord), I, I, C.c_char_p, I]
+    L.tlo_chunk_create.argtypes = [I, I, I]
+    L.tlo_chunk_create.restype = PC
+    L.tlo_chunk_destroy.argtypes = [PC]
+    L.tlo_field.argtypes = [PC, I]
+    L.tlo_field.restype = P(D)
+    L.tlo_initialise_chunk.argtypes = [PC, P(Params)]
+    L.tlo_generate_chunk.argtypes = [PC, P(Params)]
+    L.tlo_set_field.argtypes = [PC]
+    L.tlo_common_init.argtypes = [PC, P(I), I, I, I]
+    for name in ("tlo_finalise", "tlo_calc_residual", "tlo_diag_init", "tlo_block_init",
+                 "tlo_ppcg_update_z", "tlo_ppcg_store_r"):
+        getattr(L, name).argtypes = [PC]
+    for name in ("tlo_calc_2norm",):
+        getattr(L, name).argtypes = [PC, I]
+        getattr(L, name).restype = D
+    L.tlo_cg_init.argtypes = [PC, I]
+    L.tlo_cg_init.restype = D
+    L.tlo_cg_calc_w.argtypes = [PC]
+    L.tlo_cg_calc_w.restype = D
+    L.tlo_cg_calc_ur.argtypes = [PC, D, I]
+    L.tlo_cg_calc_ur.restype = D
+    L.tlo_cg_calc_p.argtypes = [PC, D, I, I]
+    L.tlo_cheby_init.argtypes = [PC, D, I]
+    L.tlo_cheby_iterate.argtypes = [PC, D, D, I]
+    L.tlo_ppcg_init_sd.argtypes = [PC, D, I]
+    L.tlo_ppcg_init.argtypes = [PC, I, I, I]
+    L.tlo_ppcg_init.restype = D
+    L.tlo_ppcg_inner.argtypes = [PC, D, D, I]
+    L.tlo_ppcg_calc_zrnorm.argtypes = [PC, I, I]
+    L.tlo_ppcg_calc_zrnorm.restype = D
+    L.tlo_ppcg_calc_rrn.argtypes = [PC]
+    L.tlo_ppcg_calc_rrn.restype = D
+    L.tlo_jacobi_solve.argtypes = [PC]
+    L.tlo_jacobi_solve.restype = D
+    L.tlo_update_halo.argtypes = [PC, P(I), I, I]
+    L.tlo_field_summary.argtypes = [PC, P(D)]
+    L.tlo_tqli.argtypes = [P(D), P(D), I]
+    L.tlo_calc_eigenvalues.argtypes = [P(D), P(D), I, P(D), P(D)]
+    L.tlo_calc_ch_coefs.argtypes = [P(D), P(D), D, D, P(D), I]
+    return L
 
 
 class OracleError(RuntimeError):
@@ -125,8 +138,9 @@ def parse_deck(text):
     return p
 
 
-def run(params, max_steps=0):
-    """Run a parsed deck; returns a list of dict records (record 0 = step 0)."""
+def run(params, max_steps=0, contracted=False):
+    """Run a parsed deck; returns a list of dict records (record 0 = step 0).
+    contracted=True runs the multiply-add build of the same source."""
     if isinstance(params, str):
         params = parse_deck(params)
     n = (params.end_step if params.end_step < 100000 else 1000) + 2
@@ -134,7 +148,7 @@ def run(params, max_steps=0):
         n = min(n, max_steps + 2)
     recs = (StepRecord * n)()
     msg = C.create_string_buffer(512)
-    rc = lib().tlo_run(C.byref(params), recs, n, max_steps, msg, 512)
+    rc = lib(contracted).tlo_run(C.byref(params), recs, n, max_steps, msg, 512)
     if rc < 0:
         raise OracleError(msg.value.decode())
     out = []
