@@ -39,7 +39,8 @@ def random_case(seed):
         kw["presteps"] = int(rng.integers(5, 31))
     if solver == "ppcg":
         kw["inner_steps"] = int(rng.integers(2, 13))
-        if rng.random() < 0.5:
+        # the reader refuses a deeper halo with the block preconditioner
+        if rng.random() < 0.5 and kw["precond"] != "jac_block":
             kw["halo_depth"] = 2
     nx, ny = (int(v) for v in rng.integers(16, 301, 2))
     if solver == "chebyshev" and seed in (1, 9):
@@ -50,7 +51,7 @@ def random_case(seed):
     return solver, nx, ny, kw
 
 
-@pytest.mark.parametrize("seed", range(16))
+@pytest.mark.parametrize("seed", range(48))
 def test_random_deck(seed, tmp_path):
     solver, nx, ny, kw = random_case(seed)
     deck = make_deck(nx, y_cells=ny, solver=solver, seed=seed, **kw)
