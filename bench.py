@@ -41,6 +41,7 @@ CG_ITER_BYTES_PER_CELL = 96.0
 # its sweeps, PPCG sweep, and the fixed work of a step (init, residual,
 # cg_init, finalise, summary)
 CHEBY_ITER_BYTES_PER_CELL = 56.0
+JACOBI_ITER_BYTES_PER_CELL = 40.0
 PPCG_OUTER_BYTES_PER_CELL = 112.0
 PPCG_SWEEP_BYTES_PER_CELL = 64.0
 STEP_FIXED_BYTES_PER_CELL = 200.0
@@ -63,6 +64,13 @@ def model_bytes_per_cell(solver, rows):
         elif solver == "ppcg":
             total += PPCG_OUTER_BYTES_PER_CELL * rest + PPCG_SWEEP_BYTES_PER_CELL * r["ppcg_inner"]
     return total
+
+
+def model_bytes_jacobi(rows):
+    """Algorithmic bytes per cell of Jacobi steps: 40 per iteration (read u
+    with its neighbours, u0, Kx, Ky; write u) plus the step's fixed work."""
+    return sum(STEP_FIXED_BYTES_PER_CELL + JACOBI_ITER_BYTES_PER_CELL * r["iterations"]
+               for r in rows)
 
 
 def measured_traffic(kernel, cells):
@@ -134,17 +142,20 @@ CLASS_BYTES = {0: ("cg calc_w", 48.0), 1: ("cg calc_ur", 48.0),
 
 
 def solver_runs(cells, steps, warmup):
-    """Chebyshev (20 CG presteps) and PPCG (10 inner steps) on the same
-    4096^2 deck at tl_eps 1e-15, tl_max_iters 10000: seconds per step,
-    iterations and the per-kernel-class HBM rates (HIP events)."""
+    """Chebyshev (20 CG presteps), PPCG (10 inner steps) and Jacobi on the
+    same 4096^2 deck at tl_eps 1e-15, tl_max_iters 10000 (Chebyshev and Jacobi
+    run to the cap): seconds per step, iterations and the per-kernel-class HBM
+    rates (HIP events)."""
     out = {}
-    for solver, kw in (("chebyshev", {"presteps": 20}), ("ppcg", {"inner_steps": 10})):
+    for solver, kw in (("chebyshev", {"presteps": 20}), ("ppcg", {"inner_steps": 10}),
+                       ("jacobi", {})):
         with tempfile.TemporaryDirectory() as wd:
             elapsed, iters, prof = gpu_run(cells, solver, steps, warmup, wd, **kw)
         rec = {"s_per_step": elapsed / steps, "iterations_per_step": sum(iters) / steps}
         if solver == "ppcg":
             rec["sweeps_per_step"] = sum(r["ppcg_inner"] for r in prof["rows"]) / steps
-        model = model_bytes_per_cell(solver, prof["rows"])
+        model = (model_bytes_jacobi(prof["rows"]) if solver == "jacobi"
+                 else model_bytes_per_cell(solver, prof["rows"]))
         rec["model_bytes_per_cell_step"] = model / steps
         rec["model_gbs"] = model * cells * cells / elapsed / 1e9
         rec["model_frac"] = rec["model_gbs"] / PEAK_HBM_GBS

@@ -69,6 +69,66 @@ __global__ __launch_bounds__(BLOCK) void k_jl(Geo G, int n, int first, double ep
   if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
 }
 
+// The same iteration on the vector walk (two columns per lane, 16-byte loads,
+// wave shuffles and LDS for the neighbours), for chunks whose external faces
+// have zero coefficients (not reflective), where the value read outside the
+// mesh is multiplied by zero (the walk mirrors, the per-cell kernel reads the
+// zero halo of un).
+template <bool NT>
+__global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double eps_abs,
+                                                const double *__restrict__ un,
+                                                double *__restrict__ uout,
+                                                const double *__restrict__ u0,
+                                                const double *__restrict__ Kx,
+                                                const double *__restrict__ Ky, double rx,
+                                                double ry, const double *__restrict__ prev_part,
+                                                int nprev, double *__restrict__ out_part, Ctl *ctl,
+                                                int dir) {
+  typedef typename dev::Vec<double, 2>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (!first) {
+    if (ctl->stop) return;
+    const double e = sqrt(fabs(dev::sum_partials(prev_part, nprev, lds)));
+    const bool conv = fabs(e) < eps_abs;
+    if (conv) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->stop = 1; ctl->reason = TL_LOOP_CONVERGED; ctl->n_end = n - 1; ctl->error = e;
+      }
+      return;
+    }
+  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->stop = 0;
+  }
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<2> tt = dev::tilev_of<2>(G, t);
+    const int cc = tt.c <= G.nx ? tt.c : (G.nx / 2) * 2;
+    auto ldop = [&](int col, int row) -> VT {
+      return dev::ldv<NT, double, 2>(un + (long)row * P + col);
+    };
+    auto ld1 = [&](int col, int row) -> double { return un[(long)row * P + col]; };
+    auto src = [&](int row) -> VT { return dev::ldv<NT, double, 2>(u0 + (long)row * P + cc); };
+    // tea_leaf_jacobi_kernel.f90:60-66, in the reference's order of operations
+    auto cell = [&](const VT &s0, int e, double c, double l, double r, double d, double u,
+                    double kxl, double kxr, double kyd, double kyu) -> double {
+      (void)c;
+      return (s0[e] + rx * (kxr * r + kxl * l) + ry * (kyu * u + kyd * d)) /
+             (1.0 + rx * (kxl + kxr) + ry * (kyd + kyu));
+    };
+    dev::walkv<NT, double, 2, false, TL_PF_W, true>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, src, [&](int y, VT v, VT w, const VT &) {
+      if (tt.c >= G.nx) return;
+      dev::stv<false, double, 2>(uout + (long)y * P, tt.c, G.nx, w);
+      acc += fabs(w.x - v.x);
+      if (tt.c + 1 < G.nx) acc += fabs(w.y - v.y);
+    }, dir, cell);
+  }
+  acc = dev::block_sum(acc, lds);
+  if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+}
+
 __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev_part, int nprev,
                           Ctl *ctl) {
   __shared__ double lds[BLOCK / WAVE];
@@ -512,17 +572,32 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   if (rc) return rc;
   if ((rc = ensure_loop_buffers(st->n_last))) return rc;
   State &s = S();
-  const Geo G = geo(BLOCK);
-  const int nb = G.blocks;
+  // the vector walk where every external face coefficient is zero (all faces
+  // external, not reflective); the per-cell walk otherwise, which reads the
+  // halo of un as the reference does (never written: zero)
+  const bool vec = s.zb_all && !s.reflective;
+  const Geo G = vec ? geo(2 * BLOCK) : geo(BLOCK);
+  const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int u0c = s.ucur;
   Ctl *ctl = (Ctl *)s.ctl;
+  auto kv = use_nontemporal() ? k_jl_v<true> : k_jl_v<false>;
   auto launch = [&](int n, bool first) -> int {
     const int in = u0c ^ ((n - st->n_first) & 1);
     hipEvent_t ev;
     prof_begin(4, &ev);
-    hipLaunchKernelGGL(k_jl, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, st->eps_abs,
-                       s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry,
-                       part((n - 1) & 1 ? P4 : P5), nb, part(n & 1 ? P4 : P5), ctl);
+    if (vec) {
+      // consecutive sweeps walk their rows in opposite directions
+      const int dir = ((n - st->n_first) & 1) ? -1 : 1;
+      hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
+                         st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
+                         s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb, part(n & 1 ? P4 : P5), ctl,
+                         dir);
+    } else {
+      hipLaunchKernelGGL(k_jl, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
+                         st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
+                         s.Ky.p11, s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb,
+                         part(n & 1 ? P4 : P5), ctl);
+    }
     prof_end(4, ev);
     TL_HIP(hipGetLastError());
     return TL_OK;

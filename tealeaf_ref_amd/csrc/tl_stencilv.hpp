@@ -10,6 +10,8 @@
 // are always accumulated in FP64 by the callers.
 #pragma once
 
+#include <type_traits>
+
 #include "tl_stencil.hpp"
 
 namespace tl {
@@ -89,6 +91,12 @@ __device__ inline TileV<V> tilev_of(const Geo &g, long t) {
 // Per-row data a walk body needs besides the operand (none).
 struct NoExtra {};
 
+// The walk's value per cell: A applied to the operand (the default), or a
+// cell rule cell(extra, e, c, l, r, d, u, kl, kr, kd, ku) of the extra data of
+// the row, the column within the lane's vector, the operand at the cell and
+// its four neighbours, and the four face coefficients.
+struct ApplyA {};
+
 // ldv_op(col, row) -> V-vector of operand values; ld1_op(col, row) -> one
 // value (only for the two cells outside the wave); extra(row) -> the other
 // per-row loads of the body.  body(y, v, w, e) gets the operand vector, A
@@ -105,10 +113,10 @@ struct NoExtra {};
 // loaded again; only the block's outer edges are loaded.  All waves of the
 // block must walk the same rows.
 template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, typename LDV,
-          typename LD1, typename EXTRA, typename BODY>
+          typename LD1, typename EXTRA, typename BODY, typename CELL = ApplyA>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
-                             EXTRA extra, BODY body, int dir = 1) {
+                             EXTRA extra, BODY body, int dir = 1, CELL cell = CELL()) {
   typedef typename Vec<T, V>::type VT;
   typedef decltype(extra(0)) E;
   constexpr int NW = BLOCK / WAVE;
@@ -194,8 +202,11 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
       else r = (col + 1 < nx) ? right : vc[e];
       const T kl = kx[e];
       const T kr = (e + 1 < V) ? kx[e + 1] : kxr;
-      w[e] = DIFF ? apply_diff<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry)
-                  : apply_t<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry);
+      if constexpr (std::is_same<CELL, ApplyA>::value)
+        w[e] = DIFF ? apply_diff<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry)
+                    : apply_t<T>(vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e], rx, ry);
+      else
+        w[e] = cell(ex, e, vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e]);
     }
     body(y, vc, w, ex);
     if (!PF && more) load_next();
