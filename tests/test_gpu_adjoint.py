@@ -158,3 +158,22 @@ def test_ppcg_adjoint(problems, prec):
         assert du <= (1e-12 if prec == "fp64" else 1e-9)
         assert rel <= (1e-10 if prec == "fp64" else 1e-7)
     assert np.array_equal(grads[0], grads[1])
+
+
+def test_reflective_boundary_matches_reference(problems):
+    """Reflective faces (the halo mirrors the cell beside it, so no flux
+    crosses an external face, as with zeroed faces): forward and gradient
+    against the same reference."""
+    theta, e0, target, win = case(23)
+    steps, rx = 3, 20.0
+    p = problems(e0, rx, rx, steps, coefficient=L.CONDUCTIVITY, eps=1e-15, checkpoint_every=2,
+                 reflective=True)
+    print("adjoint reflective", flush=True)
+    u = p.forward(theta)
+    u_ref = R.forward(theta, e0, rx, rx, steps, False)[-1]
+    assert np.max(np.abs(u - u_ref)) <= 1e-11 * np.max(np.abs(u_ref))
+    g = p.backward(seed_grad(u, target, win))
+    g_ref = R.gradient(theta, e0, rx, rx, steps, False, seed_grad(u_ref, target, win))
+    err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
+    print(f"  max rel gradient error vs reference {err:.2e}", flush=True)
+    assert err <= 1e-9
