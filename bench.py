@@ -134,11 +134,13 @@ def gpu_run(cells, solver, steps, warmup, workdir, **deck_kw):
 
 
 # algorithmic bytes per cell of one launch of the kernel classes of
-# tl_profile_get; a PPCG sweep pass or Chebyshev pass does two sweeps or
-# iterations for the bytes of one
+# tl_profile_get; a PPCG sweep pass, Chebyshev pass or Jacobi pass does two
+# sweeps or iterations for the bytes of one (plus, for Jacobi, the store of
+# the first iteration's u)
 CLASS_BYTES = {0: ("cg calc_w", 48.0), 1: ("cg calc_ur", 48.0),
                2: ("ppcg inner sweep pass", 64.0), 3: ("chebyshev pass", 56.0),
-               4: ("jacobi", 40.0), 5: ("ppcg calc_w", 48.0), 6: ("ppcg calc_ur + store_r", 56.0)}
+               4: ("jacobi pass", 40.0), 5: ("ppcg calc_w", 48.0),
+               6: ("ppcg calc_ur + store_r", 56.0)}
 
 
 def solver_runs(cells, steps, warmup):

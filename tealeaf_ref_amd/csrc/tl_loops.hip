@@ -7,6 +7,7 @@
 
 #include "tl_loopk.hpp"
 #include "tl_cheby2.hpp"
+#include "tl_jacobi2.hpp"
 
 namespace tl {
 dev::Geo geo(int cols_per_tile);
@@ -127,6 +128,106 @@ __global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double 
   }
   acc = dev::block_sum(acc, lds);
   if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
+}
+
+// Exit test of the two iterations of the previous pass (n-2, then n-1, as the
+// reference tests after every iteration): partials of iteration n-2 in pa,
+// of n-1 in pb (pa null: the previous launch did one iteration).  Returns
+// true (and records the stop) when one of them has converged; with
+// at_limit, records the stop at n-1 in any case.
+__device__ inline bool jacobi_pair_exit(int n, double eps_abs, const double *__restrict__ pa,
+                                        const double *__restrict__ pb, int nprev, Ctl *ctl,
+                                        double *lds, bool at_limit) {
+  if (pa) {
+    const double ea = sqrt(fabs(dev::sum_partials(pa, nprev, lds)));
+    if (fabs(ea) < eps_abs) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->stop = 1; ctl->reason = TL_LOOP_CONVERGED; ctl->n_end = n - 2; ctl->error = ea;
+      }
+      return true;
+    }
+  }
+  const double eb = sqrt(fabs(dev::sum_partials(pb, nprev, lds)));
+  const bool conv = fabs(eb) < eps_abs;
+  if (conv || at_limit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ctl->stop = 1;
+      ctl->reason = conv ? TL_LOOP_CONVERGED : TL_LOOP_MAXITERS;
+      ctl->n_end = n - 1;
+      ctl->error = eb;
+    }
+    return true;
+  }
+  return false;
+}
+
+// Jacobi iterations n and n+1 in one pass (tl_jacobi2.hpp): exit test of the
+// previous pass first; u of iteration n to umid, of n+1 to uout, the
+// partials of sum |u_n - u_n-1| to part_a and of sum |u_n+1 - u_n| to part_b.
+#ifndef TL_JACOBI_PAIR_WAVES
+#define TL_JACOBI_PAIR_WAVES 4
+#endif
+
+template <bool NT>
+__global__ __launch_bounds__(BLOCK, TL_JACOBI_PAIR_WAVES) void k_jl_pair(
+    Geo G, int n, int first, double eps_abs, const double *__restrict__ un,
+    double *__restrict__ umid, double *__restrict__ uout, const double *__restrict__ u0,
+    const double *__restrict__ Kx, const double *__restrict__ Ky, double rx, double ry,
+    const double *__restrict__ prev_a, const double *__restrict__ prev_b, int nprev,
+    double *__restrict__ part_a, double *__restrict__ part_b, Ctl *ctl, int dir) {
+  typedef typename dev::Vec<double, 2>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (!first) {
+    if (ctl->stop) return;
+    if (jacobi_pair_exit(n, eps_abs, prev_a, prev_b, nprev, ctl, lds, false)) return;
+  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->stop = 0;
+  }
+  const long P = G.pitch;
+  double acc_a = 0.0, acc_b = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<2> tt = dev::tilev_of<2>(G, t);
+    auto ldop = [&](int col, int row) -> VT {
+      return dev::ldv<NT, double, 2>(un + (long)row * P + col);
+    };
+    auto ld1 = [&](int col, int row) -> double { return un[(long)row * P + col]; };
+    auto src = [&](int col, int row) -> VT {
+      return dev::ldv<NT, double, 2>(u0 + (long)row * P + col);
+    };
+    auto src1 = [&](int col, int row) -> double { return u0[(long)row * P + col]; };
+    dev::walk_jacobi_pair<NT, 2>(G, tt, Kx, Ky, rx, ry, ldop, ld1, src, src1,
+                                 [&](int y, VT v0, VT v1, VT v2) {
+      if (tt.c >= G.nx) return;
+      const long row = (long)y * P;
+      dev::stv<false, double, 2>(umid + row, tt.c, G.nx, v1);
+      dev::stv<false, double, 2>(uout + row, tt.c, G.nx, v2);
+      acc_a += fabs(v1.x - v0.x);
+      acc_b += fabs(v2.x - v1.x);
+      if (tt.c + 1 < G.nx) {
+        acc_a += fabs(v1.y - v0.y);
+        acc_b += fabs(v2.y - v1.y);
+      }
+    }, dir);
+  }
+  acc_a = dev::block_sum(acc_a, lds);
+  acc_b = dev::block_sum(acc_b, lds);
+  if (threadIdx.x == 0) {
+    part_a[blockIdx.x] = acc_a;
+    part_b[blockIdx.x] = acc_b;
+  }
+}
+
+// The exit test of the launch before iteration n, by one block: before a
+// closing single iteration (at_limit 0), or after the last launch of the loop
+// (at_limit 1: a stop in any case; pa null when that launch did one
+// iteration).
+__global__ void k_jl_pair_test(int n, double eps_abs, const double *__restrict__ pa,
+                               const double *__restrict__ pb, int nprev, Ctl *ctl,
+                               int at_limit) {
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl->stop) return;
+  jacobi_pair_exit(n, eps_abs, pa, pb, nprev, ctl, lds, at_limit != 0);
 }
 
 __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev_part, int nprev,
@@ -566,6 +667,81 @@ extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const doubl
                                   inner_iters);
 }
 
+namespace {
+// Jacobi iterations two per pass (k_jl_pair).  Launch j covers iterations
+// n = n_first + 2j and n + 1, reading u from u[in_j] and writing iteration n
+// to the spare p buffer ("mid") and n + 1 to u[1 - in_j]; an odd count ends
+// with one k_jl_v iteration.  The partials of launch j go to P2/P3 (even j)
+// or P4/P5 (odd j), so a launch never overwrites what the next one reads
+// while an earlier block still reads it.  On exit at the first iteration of
+// a pass, its u is copied from mid into the u buffer the loop leaves current.
+int jacobi_loop_pairs(tl_loop_state *st) {
+  State &s = S();
+  const Geo G = geo(2 * BLOCK);
+  const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
+  const int c0 = s.ucur;
+  double *mid = s.p[1 - s.pcur].p11;  // not visible through TL_FIELD_P
+  Ctl *ctl = (Ctl *)s.ctl;
+  const bool nt = use_nontemporal();
+  auto kp = nt ? k_jl_pair<true> : k_jl_pair<false>;
+  auto kv = nt ? k_jl_v<true> : k_jl_v<false>;
+  const int n_first = st->n_first, n_last = st->n_last;
+  auto slot_a = [](int j) { return part((j & 1) ? P4 : P2); };
+  auto slot_b = [](int j) { return part((j & 1) ? P5 : P3); };
+  auto is_single = [&](int j) { return n_first + 2 * j == n_last; };
+  auto launch = [&](int n, bool first) -> int {
+    const int off = n - n_first;
+    if (off & 1) return TL_OK;  // the second iteration of the pair at n - 1
+    const int j = off / 2, in = c0 ^ (j & 1);
+    const double *pa = first ? nullptr : slot_a(j - 1), *pb = first ? nullptr : slot_b(j - 1);
+    const int dir = (j & 1) ? -1 : 1;
+    hipEvent_t ev;
+    prof_begin(4, &ev);
+    if (!is_single(j)) {
+      hipLaunchKernelGGL(kp, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
+                         st->eps_abs, s.u[in].p11, mid, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
+                         s.Ky.p11, s.rx, s.ry, pa, pb, nb, slot_a(j), slot_b(j), ctl, dir);
+    } else if (first) {
+      hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 1, st->eps_abs,
+                         s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx,
+                         s.ry, slot_b(j), nb, slot_b(j), ctl, dir);
+    } else {
+      // exit test of the previous pair, then one iteration
+      hipLaunchKernelGGL(k_jl_pair_test, dim3(1), dim3(BLOCK), 0, s.stream, n, st->eps_abs, pa,
+                         pb, nb, ctl, 0);
+      hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, st->eps_abs,
+                         s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx,
+                         s.ry, pb, nb, slot_b(j), ctl, dir);
+    }
+    prof_end(4, ev);
+    TL_HIP(hipGetLastError());
+    return TL_OK;
+  };
+  auto tail = [&](int n) -> int {
+    const int j = (n_last - n_first) / 2;  // the last launch
+    hipLaunchKernelGGL(k_jl_pair_test, dim3(1), dim3(BLOCK), 0, s.stream, n, st->eps_abs,
+                       is_single(j) ? nullptr : slot_a(j), slot_b(j), nb, ctl, 1);
+    TL_HIP(hipGetLastError());
+    return TL_OK;
+  };
+  Ctl res;
+  int rc;
+  if ((rc = drive(n_first, n_last, launch, tail, &res))) return rc;
+  st->n_end = res.n_end;
+  st->reason = res.reason;
+  st->error = res.error;
+  const int off = res.n_end - n_first, j = off / 2, in = c0 ^ (j & 1);
+  s.ucur = 1 - in;  // launch j wrote its last iteration to u[1 - in_j]
+  if ((off & 1) == 0 && !is_single(j)) {
+    // the loop ended at the first iteration of pair j, which is in mid
+    TL_HIP(hipMemcpy2DAsync(s.u[s.ucur].p11, s.pitch * sizeof(double), mid,
+                            s.pitch * sizeof(double), s.nx * sizeof(double), s.ny,
+                            hipMemcpyDeviceToDevice, s.stream));
+  }
+  return TL_OK;
+}
+}  // namespace
+
 extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   TL_REQUIRE_INIT();
   int rc = check_state(st);
@@ -576,6 +752,7 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   // external, not reflective); the per-cell walk otherwise, which reads the
   // halo of un as the reference does (never written: zero)
   const bool vec = s.zb_all && !s.reflective;
+  if (vec && sweep_pairs()) return jacobi_loop_pairs(st);
   const Geo G = vec ? geo(2 * BLOCK) : geo(BLOCK);
   const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int u0c = s.ucur;

@@ -314,3 +314,36 @@ def test_profile_sampling():
     assert e1 == e4 and n1 > 8
     assert n4 == (n1 + 3) // 4, (n1, n4)
     assert ms1 > 0.0 and ms4 > 0.0
+
+
+@pytest.mark.parametrize("nx,ny", [(45, 50), (1100, 70), (300, 1400)])
+def test_jacobi_pairs_match_singles(nx, ny, monkeypatch):
+    """Two Jacobi iterations per pass (with both exit tests per pass, the
+    first iteration's u kept for an exit there) give what one iteration per
+    pass gives: the same exit iteration and reason, u to the last bits, for
+    exits at either iteration of a pair and for odd and even limits."""
+    cases = [(1e-3, 499), (3e-4, 499), (1e-30, 7), (1e-30, 8), (1e-30, 1)]
+    got = {}
+    for pairs in ("0", "1"):
+        monkeypatch.setenv("TL_SWEEP_PAIRS", pairs)
+        for eps_abs, n_last in cases:
+            print(f"jacobi pairs={pairs} {nx}x{ny} eps {eps_abs} n_last {n_last}", flush=True)
+            p = Pair(nx, ny, 1, seed=53)
+            try:
+                p.init()
+                st = L.LoopState(n_first=1, n_last=n_last, eps_abs=eps_abs)
+                L.call("tl_jacobi_loop", C.byref(st))
+                got[(pairs, eps_abs, n_last)] = (st.n_end, st.reason, st.error, p.d.download("u"))
+            finally:
+                p.close()
+    parities = set()
+    for eps_abs, n_last in cases:
+        (n0, r0, e0, u0), (n1, r1, e1, u1) = got[("0", eps_abs, n_last)], got[("1", eps_abs, n_last)]
+        assert (n0, r0) == (n1, r1), (eps_abs, n_last, n0, r0, n1, r1)
+        assert abs(e0 - e1) <= 1e-9 * abs(e0), (e0, e1)
+        d = np.max(np.abs(u0 - u1)) / np.max(np.abs(u0))
+        print(f"  eps {eps_abs} n_last {n_last}: exit {n0} reason {r0}, u max rel diff {d:.1e}",
+              flush=True)
+        assert d <= 1e-12, d
+        parities.add(n0 % 2)
+    assert parities == {0, 1}
