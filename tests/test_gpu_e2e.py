@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 RTOL = 1e-10
 # the decks the reference itself ships (tea_bm_1..4) and every captured one;
-# tea_bm_5/6 (4000^2, 8000^2) are run by the benchmark-scale test below
+# tea_bm_5/6 (4000^2, 8000^2) are run by test_benchmark_scale_deck below
 SMALL = [n for n in FX.names() if n not in ("tea_bm_5", "tea_bm_6")]
 
 
@@ -43,3 +43,32 @@ def test_tea_leaf_program(tmp_path):
     FX.compare_summaries(got["summaries"], fx["summaries"], RTOL)
     FX.compare_iterations(got["steps"], fx["steps"])
     assert "This test is considered PASSED" in (tmp_path / "tea.out").read_text()
+
+
+@pytest.mark.timeout(400)
+@pytest.mark.parametrize("name", ["tea_bm_5", "tea_bm_6"])
+def test_benchmark_scale_deck(name, tmp_path):
+    """The reference's own benchmark decks at scale (4000^2 and 8000^2 CG,
+    10 steps, tl_eps 1e-15), against the logs it ships, which came from 24
+    and 36 MPI ranks: per-step summaries within 1e-10; iteration counts
+    within 0.1 % (at least 1).  At this tolerance the exit sits on the FP64
+    floor: the shipped logs end every step with a conduction error of
+    0.993-0.999e-15, i.e. the residual falls by under 1 % per iteration there,
+    so rounding noise of the residual at the per-cent level (another
+    reduction order, as between 36 ranks and one GPU) moves the exit by a few
+    iterations."""
+    import time
+    fx = FX.load(name)
+    print(f"[e2e] {name}", flush=True)
+    t0 = time.time()
+    got = host.run_deck(fx["deck"], str(tmp_path))
+    wall = time.time() - t0
+    worst = FX.compare_summaries(got["summaries"], fx["summaries"], RTOL)
+    diffs = {int(k): got["steps"][int(k)]["iterations"] - r["iterations"]
+             for k, r in fx["steps"].items()}
+    total = sum(got["steps"][int(k)]["iterations"] for k in fx["steps"])
+    print(f"  {name}: {wall:.1f} s for {total} CG iterations, worst summary rel diff "
+          f"{worst:.1e}, iteration differences per step {diffs}", flush=True)
+    for k, d in diffs.items():
+        ref = fx["steps"][str(k)]["iterations"]
+        assert abs(d) <= max(FX.iteration_slack(fx["deck"]), int(1e-3 * ref)), (k, d, ref)
