@@ -174,6 +174,35 @@ def solver_runs(cells, steps, warmup):
     return out
 
 
+def stream_bandwidth(n=1 << 28, reps=20):
+    """Achievable HBM bandwidth on this GPU, STREAM style, with torch's
+    elementwise kernels on FP64 arrays of n elements (2 GiB each, far beyond
+    the 256 MB Infinity Cache): copy (16 B per element) and triad
+    c = a + s b (24 B per element), timed with events over `reps` launches."""
+    import torch
+    dev = torch.device("cuda", 0)
+    a = torch.rand(n, dtype=torch.float64, device=dev)
+    b = torch.rand(n, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    out = {"elements": n, "dtype": "f64"}
+    for name, fn, nbytes in (("copy", lambda: c.copy_(a), 16.0),
+                             ("triad", lambda: torch.add(a, b, alpha=3.0, out=c), 24.0)):
+        fn()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            fn()
+        t1.record()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1) / reps
+        out[name + "_gbs"] = nbytes * n / (ms * 1e-3) / 1e9
+    del a, b, c
+    torch.cuda.empty_cache()
+    log(f"[bench] stream bandwidth: {out}")
+    return out
+
+
 def small_run(steps=10):
     """The reference's CPU-sized case: 512x512, 3 seeded rectangles (seed 1),
     FP64 CG, tl_eps 1e-15, 10 steps; seconds per step and per iteration."""
@@ -503,6 +532,8 @@ def main():
     ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
                     help="true relative residual of the mixed-precision solves")
     ap.add_argument("--no-small", action="store_true", help="skip the 512^2 CG case")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the STREAM-style copy/triad bandwidth measurement")
     ap.add_argument("--solver-steps", type=int, default=1,
                     help="timed steps of the Chebyshev and PPCG runs (0 to skip)")
     ap.add_argument("--mixed-steps", type=int, default=2)
@@ -566,6 +597,10 @@ def main():
         out["solvers"] = solver_runs(args.cells, args.solver_steps, 1)
     if not args.no_small:
         out["small"] = small_run()
+    if not args.no_stream:
+        out["achievable_hbm"] = stream_bandwidth()
+        out["roofline"]["frac_of_achievable_triad"] = (
+            w_gbs / out["achievable_hbm"]["triad_gbs"] if w_gbs else None)
     sizes = [int(x) for x in args.mixed_sizes.split(",") if x.strip()]
     if sizes:
         out["precision_comparison"] = [
