@@ -123,7 +123,9 @@ constexpr bool diff_form() {
 // Streaming hints of the CG and PPCG passes (measured at 4096^2).  u is
 // touched only by the update pass, so its store streams past the Infinity
 // Cache (TL_ST_NT_U, TL_ST_NT_PPU), which then still holds the r and p the
-// next w pass reads: the CG w pass drops from 141 to about 126 us.  Plain
+// next w pass reads: the CG w pass drops from 141 to about 126 us (FP64
+// only: the FP32 fields of the mixed solver are half the size and its CG
+// measured 2 % slower with the streaming store).  Plain
 // loads of the w pass's operand (TL_LD_NT_OPND=0) or of p and w in the
 // update pass (TL_LD_NT_UR=0), and streaming r_store (TL_ST_NT_RSTORE),
 // measured slower or no faster.
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
       const long i = (long)y * P + tt.c;
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<(NT && TL_LD_NT_UR), T, V>(p + i);
       const VT rv = dev::ldv<NT, T, V>(r + i) - a * dev::ldv<(NT && TL_LD_NT_UR), T, V>(w + i);
-      dev::stv<(NT && (TL_ST_NT_LOOP || TL_ST_NT_U)), T, V>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::stv<(NT && (TL_ST_NT_LOOP || (TL_ST_NT_U && sizeof(T) == 8))), T, V>(
+          u + (long)y * P, tt.c, G.nx, uv);
       dev::stv<(NT && TL_ST_NT_LOOP), T, V>(r + (long)y * P, tt.c, G.nx, rv);
       if (diag) {
         const VT zv = dev::ldv<NT, T, V>(Mi + i) * rv;
@@ -418,7 +421,8 @@ __global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
       const VT ro = dev::ldv<NT, T, V>(r + i);
       const VT rv = ro - a * dev::ldv<NT, T, V>(w + i);
-      dev::stv<(NT && (TL_ST_NT_LOOP || TL_ST_NT_PPU)), T, V>(u + (long)y * P, tt.c, G.nx, uv);
+      dev::stv<(NT && (TL_ST_NT_LOOP || (TL_ST_NT_PPU && sizeof(T) == 8))), T, V>(
+          u + (long)y * P, tt.c, G.nx, uv);
       dev::stv<(NT && (TL_ST_NT_LOOP || TL_ST_NT_RSTORE)), T, V>(r_store + (long)y * P, tt.c,
                                                                   G.nx, ro);
       dev::stv<(NT && TL_ST_NT_LOOP), T, V>(r + (long)y * P, tt.c, G.nx, rv);
