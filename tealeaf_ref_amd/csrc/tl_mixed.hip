@@ -295,13 +295,14 @@ int refine(bool with_d, double *rr64, double *rr32) {
   }
   prof_end(7, ev);
   TL_HIP(hipGetLastError());
-  if (with_d) s.ucur = 1 - s.ucur;
   const int slots[2] = {P4, P5};
   double out[2];
   int rc = finish_sums(slots, 2, nb, out);
   if (rc) return rc;
+  // a non-finite result leaves u as it was (the new u went to the other buffer)
   if (!std::isfinite(out[0]) || !std::isfinite(out[1]))
     return fail(TL_ERR_NONFINITE, "mixed precision: non-finite residual");
+  if (with_d) s.ucur = 1 - s.ucur;
   *rr64 = out[0];
   *rr32 = out[1];
   return TL_OK;
