@@ -282,6 +282,10 @@ def precision_comparison(cells, steps, warmup, inner_steps=10):
             rec["sweeps_per_step"] = sum(r["ppcg_inner"] for r in rows) / steps
         if extra:
             rec["refinements_per_step"] = sum(r["refinements"] for r in rows) / steps
+        else:
+            # HBM rate in the algorithmic bytes of the reference's dataflow
+            rec["model_gbs"] = (model_bytes_per_cell(solver, rows) * cells * cells / elapsed
+                                / 1e9)
         summ = [last["volume"], last["mass"], last["energy"], last["temperature"]]
         if ref_u is None:
             ref_u, ref_sum = u, summ
