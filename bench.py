@@ -73,12 +73,13 @@ def model_bytes_jacobi(rows):
                for r in rows)
 
 
-def measured_traffic(kernel, cells):
-    """HBM bytes per launch of a kernel from the committed counter summary
-    (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated), or None."""
+def measured_traffic(kernel, cells, profile="cg_fp64_4096_hbm_traffic.md"):
+    """HBM bytes per launch of a kernel at 4096^2 from a committed counter
+    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated), or
+    None."""
     if cells != 4096:
         return None
-    path = os.path.join(ROOT, "profiles", "cg_fp64_4096_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", profile)
     try:
         text = open(path).read()
     except OSError:
@@ -126,8 +127,10 @@ def gpu_run(cells, solver, steps, warmup, workdir, **deck_kw):
     for cls in range(8):
         n = C.c_long()
         ms = C.c_double()
+        idle = C.c_long()
         _lib.call("tl_profile_get", cls, C.byref(n), C.byref(ms))
-        prof[cls] = (n.value, ms.value)
+        _lib.call("tl_profile_idle", cls, C.byref(idle))
+        prof[cls] = (n.value, ms.value, idle.value)
     _lib.call("tl_profile_enable", 0)
     h.close()
     return elapsed, iters, prof
@@ -139,8 +142,17 @@ def gpu_run(cells, solver, steps, warmup, workdir, **deck_kw):
 # the first iteration's u)
 CLASS_BYTES = {0: ("cg calc_w", 48.0), 1: ("cg calc_ur", 48.0),
                2: ("ppcg inner sweep pass", 64.0), 3: ("chebyshev pass", 56.0),
-               4: ("jacobi pass", 40.0), 5: ("ppcg calc_w", 48.0),
+               4: ("jacobi pass", 48.0), 5: ("ppcg calc_w", 48.0),
                6: ("ppcg calc_ur + store_r", 56.0)}
+# the kernel of each class and the committed FETCH_SIZE / WRITE_SIZE summary
+# that holds its measured HBM bytes per launch at 4096^2
+CLASS_TRAFFIC = {0: ("k_loop_w", "cg_fp64_4096_hbm_traffic.md"),
+                 1: ("k_cg_ur", "cg_fp64_4096_hbm_traffic.md"),
+                 2: ("k_pp_sweep2", "ppcg_fp64_4096_hbm_traffic.md"),
+                 3: ("k_cheby_pair", "cheby_fp64_4096_hbm_traffic.md"),
+                 4: ("k_jl_pair", "jacobi_fp64_4096_kernel_trace.md"),
+                 5: ("k_loop_w", "ppcg_fp64_4096_hbm_traffic.md"),
+                 6: ("k_pp_ur", "ppcg_fp64_4096_hbm_traffic.md")}
 
 
 def solver_runs(cells, steps, warmup):
@@ -163,11 +175,20 @@ def solver_runs(cells, steps, warmup):
         rec["model_frac"] = rec["model_gbs"] / PEAK_HBM_GBS
         kern = {}
         for cls, (name, bpc) in CLASS_BYTES.items():
-            n, ms = prof[cls]
+            n, ms, idle = prof[cls]
             if n:
-                kern[name] = {"timed_launches": n, "avg_launch_ms": ms / n,
-                              "achieved_gbs": bpc * cells * cells * n / (ms * 1e-3) / 1e9,
-                              "bytes_per_cell": bpc}
+                k = {"timed_launches": n, "idle_launches": idle, "avg_launch_ms": ms / n,
+                     "achieved_gbs": bpc * cells * cells * n / (ms * 1e-3) / 1e9,
+                     "bytes_per_cell": bpc}
+                kname, prof_file = CLASS_TRAFFIC[cls]
+                traffic = measured_traffic(kname, cells, prof_file)
+                if traffic:
+                    # the same launches in the HBM bytes the counters measured
+                    k["measured_bytes_per_launch"] = traffic
+                    k["measured_gbs"] = traffic / (ms / n * 1e-3) / 1e9
+                    k["measured_frac"] = k["measured_gbs"] / PEAK_HBM_GBS
+                    k["traffic_source"] = "profiles/" + prof_file
+                kern[name] = k
         rec["kernels"] = kern
         log(f"[bench] {cells}^2 {solver}: {rec}")
         out[solver] = rec
@@ -218,10 +239,12 @@ def small_run(steps=10):
     return rec
 
 
-def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
+def run_deck_timed(deck, steps, warmup, workdir, want_u=False, prof_out=None):
     """Run `warmup` + `steps` timesteps of a deck through the native host;
     returns the timed seconds, the per-step results of the timed steps and
-    the final u (interior) when asked."""
+    the final u (interior) when asked.  With a dict `prof_out`, every
+    EVENT_PERIOD-th launch of each kernel class is timed and prof_out[class]
+    = (timed launches, their milliseconds, idle launches)."""
     import ctypes as C
     import numpy as np
     from tealeaf_ref_amd import _lib
@@ -233,6 +256,9 @@ def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
     for _ in range(warmup):
         h.step()
     _lib.call("tl_synchronize")
+    if prof_out is not None:
+        _lib.call("tl_profile_reset")
+        _lib.call("tl_profile_enable", EVENT_PERIOD)
     rows = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -240,6 +266,13 @@ def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
         rows.append(h.last())
     _lib.call("tl_synchronize")
     elapsed = time.perf_counter() - t0
+    if prof_out is not None:
+        for cls in range(8):
+            n, ms, idle = C.c_long(), C.c_double(), C.c_long()
+            _lib.call("tl_profile_get", cls, C.byref(n), C.byref(ms))
+            _lib.call("tl_profile_idle", cls, C.byref(idle))
+            prof_out[cls] = (n.value, ms.value, idle.value)
+        _lib.call("tl_profile_enable", 0)
     u = None
     if want_u:
         m = re.search(r"x_cells=(\d+)", deck)
@@ -249,6 +282,36 @@ def run_deck_timed(deck, steps, warmup, workdir, want_u=False):
         u = u[1:-1, 1:-1].copy()
     h.close()
     return elapsed, rows, u
+
+
+# kernel classes of the FP32 inner PPCG of a mixed-precision solve: name,
+# algorithmic bytes per cell (FP32 fields) and the kernel's line in
+# profiles/ppcg_mixed_4096_hbm_traffic.md (the FP64 refinements, a few per
+# step, are too rare for sampled events)
+MIXED_PPCG_CLASSES = {2: ("fp32 ppcg inner sweep pass", 32.0, "k_pp_sweep2<float>"),
+                      5: ("fp32 ppcg calc_w", 24.0, "k_loop_w<float>"),
+                      6: ("fp32 ppcg calc_ur + store_r", 28.0, "k_pp_ur<float>")}
+
+
+def mixed_kernel_rates(prof, cells):
+    """Per-class launch time and HBM rates of a mixed-precision PPCG run:
+    algorithmic bytes, and the counter-measured bytes where a summary holds
+    them (4096^2)."""
+    out = {}
+    for cls, (name, bpc, kname) in MIXED_PPCG_CLASSES.items():
+        n, ms, idle = prof.get(cls, (0, 0.0, 0))
+        if not n:
+            continue
+        k = {"timed_launches": n, "idle_launches": idle, "avg_launch_ms": ms / n,
+             "achieved_gbs": bpc * cells * cells / (ms / n * 1e-3) / 1e9, "bytes_per_cell": bpc}
+        traffic = measured_traffic(kname, cells, "ppcg_mixed_4096_hbm_traffic.md")
+        if traffic:
+            k["measured_bytes_per_launch"] = traffic
+            k["measured_gbs"] = traffic / (ms / n * 1e-3) / 1e9
+            k["measured_frac"] = k["measured_gbs"] / PEAK_HBM_GBS
+            k["traffic_source"] = "profiles/ppcg_mixed_4096_hbm_traffic.md"
+        out[name] = k
+    return out
 
 
 def precision_comparison(cells, steps, warmup, inner_steps=10):
@@ -271,8 +334,10 @@ def precision_comparison(cells, steps, warmup, inner_steps=10):
         if solver == "ppcg":
             kw["inner_steps"] = inner_steps
         deck = bench_deck(cells, solver=solver, **kw)
+        prof = {} if name == "mixed_ppcg" else None
         with tempfile.TemporaryDirectory() as wd:
-            elapsed, rows, u = run_deck_timed(deck, steps, warmup, wd, want_u=True)
+            elapsed, rows, u = run_deck_timed(deck, steps, warmup, wd, want_u=True,
+                                              prof_out=prof)
         last = rows[-1]
         rec = {"s_per_step": elapsed / steps,
                "iterations_per_step": sum(r["iterations"] for r in rows) / steps,
@@ -286,6 +351,8 @@ def precision_comparison(cells, steps, warmup, inner_steps=10):
             # HBM rate in the algorithmic bytes of the reference's dataflow
             rec["model_gbs"] = (model_bytes_per_cell(solver, rows) * cells * cells / elapsed
                                 / 1e9)
+        if prof:
+            rec["kernels"] = mixed_kernel_rates(prof, cells)
         summ = [last["volume"], last["mass"], last["energy"], last["temperature"]]
         if ref_u is None:
             ref_u, ref_sum = u, summ
@@ -558,11 +625,12 @@ def main():
     s_per_step = elapsed / args.steps
     exec_iters = sum(i + 1 for i in iters)        # printed count is n-1 at exit
     mean_iters = sum(iters) / len(iters)
-    n_w, ms_w = prof[0]
-    n_ur, ms_ur = prof[1]
+    n_w, ms_w, idle_w = prof[0]
+    n_ur, ms_ur, _ = prof[1]
     # achieved = algorithmic bytes per launch / the average event-timed launch
-    # (every EVENT_PERIOD-th launch; like rocprof's average, it includes the
-    # few launches queued past the exit that return at once)
+    # (every EVENT_PERIOD-th launch; the few launches queued past the exit,
+    # which return at once, are left out: rocprof's average over all launches
+    # includes them and reads ~0.5 % lower)
     w_gbs = CG_W_BYTES_PER_CELL * cells / (ms_w / n_w * 1e-3) / 1e9 if ms_w else None
     ur_gbs = CG_UR_BYTES_PER_CELL * cells / (ms_ur / n_ur * 1e-3) / 1e9 if ms_ur else None
     step_gbs = CG_ITER_BYTES_PER_CELL * cells * exec_iters / elapsed / 1e9
@@ -593,6 +661,7 @@ def main():
                      "traffic_source": "profiles/cg_fp64_4096_hbm_traffic.md",
                      "bytes_per_cell": CG_W_BYTES_PER_CELL,
                      "avg_launch_ms": ms_w / n_w if n_w else None, "timed_launches": n_w,
+                     "idle_launches": idle_w,
                      "event_sample_period": EVENT_PERIOD,
                      "calc_ur_achieved": ur_gbs,
                      "calc_ur_avg_launch_ms": ms_ur / n_ur if n_ur else None},

@@ -307,8 +307,12 @@ int tl_profile_enable(int on);
 /* Returns timed launch count and their total milliseconds for a kernel class:
  * 0 = CG calc_w, 1 = CG calc_ur, 2 = PPCG inner sweep, 3 = Chebyshev iterate,
  * 4 = Jacobi, 5 = PPCG calc_w, 6 = PPCG calc_ur, 7 = mixed-precision FP64
- * refinement (u += d, true residual). */
+ * refinement (u += d, true residual).  Launches that returned at once (queued
+ * past a loop's exit: shorter than a quarter of the median timed launch of
+ * the class) are left out and counted by tl_profile_idle. */
 int tl_profile_get(int kernel_class, long *launches, double *total_ms);
+/* Timed launches of a kernel class left out of tl_profile_get as idle. */
+int tl_profile_idle(int kernel_class, long *launches);
 int tl_profile_reset(void);
 
 #ifdef __cplusplus

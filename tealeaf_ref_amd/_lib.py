@@ -88,6 +88,7 @@ def lib():
             "tl_snapshot_fill": [I, D], "tl_snapshot_upload": [I, PD, I],
             "tl_snapshot_download": [I, PD, I], "tl_adjoint_face_grad": [I, I, I, I],
             "tl_device_bytes": [PD], "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
+            "tl_profile_idle": [I, P(C.c_long)],
             "tl_profile_reset": [],
         }
         for name, args in sig.items():

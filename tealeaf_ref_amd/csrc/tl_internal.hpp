@@ -91,6 +91,7 @@ struct State {
   std::vector<hipEvent_t> ev_pool;
   long prof_count[8] = {0};
   double prof_ms[8] = {0};
+  long prof_idle[8] = {0};  // timed launches that returned at once
 
   std::string err;
 };

@@ -1,5 +1,6 @@
 // Lifecycle, field storage, host<->device copies, reduction finishing and
 // kernel timing of libtealeaf_hip.so.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -41,11 +42,52 @@ Grid grid_for(int nx, int ny, int cols_per_tile, int target_tiles, int max_rows)
 // --------------------------------------------------------------------------
 // kernel timing
 // --------------------------------------------------------------------------
+// Adds the pending events' durations to the per-class totals.  Launches
+// queued past a loop's exit return at once: a launch shorter than a quarter
+// of the median of its class among the pending events is one of those and is
+// counted apart (prof_idle), so the totals describe launches that did the
+// work.  (The median, not the maximum: a launch that the GPU reached before
+// the host had queued it measures the host's gap too.)
+static void prof_resolve() {
+  State &s = S();
+  (void)hipStreamSynchronize(s.stream);
+  const size_t n = s.ev_pending.size();
+  std::vector<float> ms(n, -1.f);
+  std::vector<float> by_cls[8];
+  for (size_t i = 0; i < n; ++i) {
+    const auto &ev = s.ev_pending[i];
+    if (hipEventElapsedTime(&ms[i], ev.a, ev.b) != hipSuccess) ms[i] = -1.f;
+    else by_cls[ev.cls].push_back(ms[i]);
+  }
+  float cut[8];
+  for (int c = 0; c < 8; ++c) {
+    std::vector<float> &v = by_cls[c];
+    cut[c] = 0.f;
+    if (v.empty()) continue;
+    std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+    cut[c] = 0.25f * v[v.size() / 2];
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const auto &ev = s.ev_pending[i];
+    if (ms[i] >= 0.f) {
+      if (ms[i] < cut[ev.cls]) {
+        s.prof_idle[ev.cls] += 1;
+      } else {
+        s.prof_ms[ev.cls] += ms[i];
+        s.prof_count[ev.cls] += 1;
+      }
+    }
+    s.ev_pool.push_back(ev.a);
+    s.ev_pool.push_back(ev.b);
+  }
+  s.ev_pending.clear();
+}
+
 void prof_begin(int cls, hipEvent_t *a) {
   State &s = S();
   *a = nullptr;
-  if (!s.prof) return;
-  if (cls >= 0 && cls < 8 && (s.prof_seq[cls]++ % s.prof_period) != 0) return;
+  if (!s.prof || cls < 0 || cls >= 8) return;
+  if ((s.prof_seq[cls]++ % s.prof_period) != 0) return;
   if (s.ev_pool.empty()) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -69,35 +111,11 @@ void prof_end(int cls, hipEvent_t a) {
   (void)hipEventRecord(b, s.stream);
   s.ev_pending.push_back({a, b, cls});
   // keep the number of outstanding events bounded
-  if (s.ev_pending.size() >= 4096) {
-    (void)hipStreamSynchronize(s.stream);
-    for (auto &ev : s.ev_pending) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) {
-        s.prof_ms[ev.cls] += ms;
-        s.prof_count[ev.cls] += 1;
-      }
-      s.ev_pool.push_back(ev.a);
-      s.ev_pool.push_back(ev.b);
-    }
-    s.ev_pending.clear();
-  }
+  if (s.ev_pending.size() >= 4096) prof_resolve();
 }
 
 static void prof_drain() {
-  State &s = S();
-  if (s.ev_pending.empty()) return;
-  (void)hipStreamSynchronize(s.stream);
-  for (auto &ev : s.ev_pending) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess) {
-      s.prof_ms[ev.cls] += ms;
-      s.prof_count[ev.cls] += 1;
-    }
-    s.ev_pool.push_back(ev.a);
-    s.ev_pool.push_back(ev.b);
-  }
-  s.ev_pending.clear();
+  if (!S().ev_pending.empty()) prof_resolve();
 }
 
 // --------------------------------------------------------------------------
@@ -339,6 +357,14 @@ extern "C" int tl_profile_get(int cls, long *launches, double *total_ms) {
   return TL_OK;
 }
 
+extern "C" int tl_profile_idle(int cls, long *launches) {
+  TL_REQUIRE_INIT();
+  if (cls < 0 || cls >= 8) return fail(TL_ERR_ARG, "bad kernel class");
+  prof_drain();
+  if (launches) *launches = S().prof_idle[cls];
+  return TL_OK;
+}
+
 extern "C" int tl_profile_reset(void) {
   TL_REQUIRE_INIT();
   prof_drain();
@@ -346,6 +372,7 @@ extern "C" int tl_profile_reset(void) {
     S().prof_count[i] = 0;
     S().prof_ms[i] = 0.0;
     S().prof_seq[i] = 0;
+    S().prof_idle[i] = 0;
   }
   return TL_OK;
 }

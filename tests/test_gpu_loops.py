@@ -305,14 +305,18 @@ def test_profile_sampling():
                              old_error=r0)
             L.call("tl_cg_loop", C.byref(st), None, None)
             L.call("tl_profile_enable", 0)
-            n, ms = C.c_long(), C.c_double()
+            n, ms, idle = C.c_long(), C.c_double(), C.c_long()
             L.call("tl_profile_get", 0, C.byref(n), C.byref(ms))
-            counts.append((n.value, ms.value, st.n_end))
+            L.call("tl_profile_idle", 0, C.byref(idle))
+            counts.append((n.value + idle.value, ms.value, st.n_end, n.value))
         finally:
             p.close()
-    (n1, ms1, e1), (n4, ms4, e4) = counts
+    (n1, ms1, e1, a1), (n4, ms4, e4, a4) = counts
     assert e1 == e4 and n1 > 8
+    # launches timed, with or without work
     assert n4 == (n1 + 3) // 4, (n1, n4)
+    # every launch up to the exit did work and is kept
+    assert a1 >= e1 and a4 >= (e1 + 3) // 4 - 1, (a1, a4, e1)
     assert ms1 > 0.0 and ms4 > 0.0
 
 
