@@ -29,6 +29,16 @@ def test_calls_fail_cleanly_without_init():
     assert b"tl_init" in L.tl_last_error()
 
 
+def test_init_rejects_bad_sizes_before_any_device_call():
+    """Size checks come before the first HIP call, so they hold here too."""
+    L = _lib.lib()
+    for args in ((0, 10, 1, 0), (10, -1, 1, 0), (10, 10, 0, 0), (10, 10, 33, 0)):
+        assert L.tl_init(*args) == 3, args  # TL_ERR_ARG
+        assert b"tl_init" in L.tl_last_error()
+    n = ctypes.c_long()
+    assert L.tl_profile_idle(0, ctypes.byref(n)) == 2  # TL_ERR_STATE
+
+
 def test_host_library_exports():
     L = host.host_lib()
     for n in ("tea_host_init", "tea_host_step", "tea_host_run", "tea_host_last",
