@@ -42,6 +42,7 @@ struct Field {
 struct State {
   bool init = false;
   int nx = 0, ny = 0, h = 1, device = 0;
+  int cus = 0;             // compute units of the device
   long pitch = 0;          // doubles per row
   long rows = 0;           // ny + 2h
   size_t field_bytes = 0;

@@ -39,6 +39,33 @@ dev::Geo geo(int cols_per_tile) {
   return G;
 }
 
+// Tiles of cols_per_tile columns, as many as blocks_per_cu blocks on every
+// CU (all resident at once for a kernel that fits blocks_per_cu blocks per
+// CU): rows per tile rounded up, so the count does not pass that number.
+dev::Geo geo_resident(int cols_per_tile, int blocks_per_cu) {
+  State &s = S();
+  const long want = (long)blocks_per_cu * (s.cus > 0 ? s.cus : 256);
+  Grid g;
+  g.tiles_x = (s.nx + cols_per_tile - 1) / cols_per_tile;
+  long r = ((long)g.tiles_x * s.ny + want - 1) / want;
+  if (r < 1) r = 1;
+  if (r > 128) r = 128;
+  g.rows_per_tile = (int)r;
+  g.tiles_y = (s.ny + g.rows_per_tile - 1) / g.rows_per_tile;
+  g.tiles = (long)g.tiles_x * g.tiles_y;
+  long b = g.tiles < MAX_GRID ? g.tiles : MAX_GRID;
+  if (b >= NUM_XCD) b -= b % NUM_XCD;
+  dev::Geo G;
+  G.nx = s.nx;
+  G.ny = s.ny;
+  G.pitch = s.pitch;
+  G.tiles_x = g.tiles_x;
+  G.rows_per_tile = g.rows_per_tile;
+  G.tiles = g.tiles;
+  G.blocks = (int)b;
+  return G;
+}
+
 }  // namespace tl
 
 using namespace tl;

@@ -11,6 +11,7 @@
 
 namespace tl {
 dev::Geo geo(int cols_per_tile);
+dev::Geo geo_resident(int cols_per_tile, int blocks_per_cu);
 }
 using namespace tl;
 using dev::Geo;
@@ -336,11 +337,12 @@ __global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_v(Geo G, const 
 // Chebyshev iterations k and k+1 in one pass (tl_cheby2.hpp): u and p are
 // read from uin / pin and written to uout / pout.  On the last pass of a batch
 // w, r (and z) of iteration k+1 are stored and the partials of r.r written.
-// compiled for 4 waves per SIMD, so that a 4096^2 grid (1024 blocks of 4
-// waves) is resident at once; this spills 29 registers, and 3 waves (no
-// spills, a quarter of the blocks in a second round) measured the same
+// Compiled for 3 waves per SIMD (160 registers, no spills), on a grid of
+// 3 blocks per CU so that every block is resident at once (cheby_geo): at
+// 4096^2 179.6 us per pass, against 205.6 with 4 waves (29 registers
+// spilled) on 1024 blocks and 213.7 with 3 waves on 1536.
 #ifndef TL_CHEBY_PAIR_WAVES
-#define TL_CHEBY_PAIR_WAVES 4
+#define TL_CHEBY_PAIR_WAVES 3
 #endif
 
 template <bool NT>
@@ -422,7 +424,7 @@ int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_fi
                     int want_norm, double *norm) {
   State &s = S();
   if (s.reflective || !s.zb_all || s.precond == TL_PREC_JAC_BLOCK) return 0;
-  const Geo G = geo(2 * BLOCK);
+  const Geo G = geo_resident(2 * BLOCK, TL_CHEBY_PAIR_WAVES);
   const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int diag = s.precond == TL_PREC_JAC_DIAG;
   const bool nt = use_nontemporal();

@@ -208,6 +208,7 @@ extern "C" int tl_init(int nx, int ny, int halo, int device) {
   TL_HIP(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(TL_ERR_ARG, "tl_init: no such HIP device");
   TL_HIP(hipSetDevice(device));
+  TL_HIP(hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, device));
   s.nx = nx;
   s.ny = ny;
   s.h = halo;
