@@ -1,0 +1,45 @@
+"""bench.py on a machine without a GPU: the figures it takes from the
+committed counter summaries are there and consistent with the algorithmic
+bytes, and the multi-GPU request is refused with a message."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+CELLS = 4096 * 4096
+
+
+def test_every_kernel_class_has_measured_traffic():
+    for cls, (kernel, profile) in bench.CLASS_TRAFFIC.items():
+        t = bench.measured_traffic(kernel, 4096, profile)
+        assert t, (cls, kernel, profile)
+        name, bpc = bench.CLASS_BYTES[cls]
+        # the counters see at least the algorithmic bytes, and ghost rows,
+        # columns and the extra loads of a batch's last pass add under 20 %
+        assert bpc * CELLS * 0.99 <= t <= bpc * CELLS * 1.2, (name, t / CELLS, bpc)
+
+
+@pytest.mark.parametrize("cls", sorted(bench.MIXED_PPCG_CLASSES))
+def test_mixed_classes_have_measured_traffic(cls):
+    name, bpc, kernel = bench.MIXED_PPCG_CLASSES[cls]
+    t = bench.measured_traffic(kernel, 4096, "ppcg_mixed_4096_hbm_traffic.md")
+    assert t and bpc * CELLS * 0.99 <= t <= bpc * CELLS * 1.2, (name, t)
+
+
+def test_roofline_traffic_of_the_dominant_kernel():
+    t = bench.measured_traffic("k_loop_w", 4096)
+    assert t and abs(t / CELLS - bench.CG_W_BYTES_PER_CELL) < 3.0
+    assert bench.measured_traffic("k_loop_w", 2048) is None  # only the profiled size
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
