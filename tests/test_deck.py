@@ -56,6 +56,9 @@ def test_unknown_keywords_ignored():
     ("halo_depth=2\ntl_preconditioner_type jac_block", "nonstandard halo depth"),
     ("x_cells=abc", "convert to integer"),
     ("state 2 density=1.0", "State defined twice"),
+    # parse.f90:343 appends ".0" to a number without a point, so an exponent
+    # needs a point in its mantissa
+    ("tl_eps 1e-5", "convert to real"),
 ])
 def test_errors(extra, msg):
     with pytest.raises(O.OracleError, match=msg):
