@@ -1,7 +1,6 @@
 """End to end: the native host (Fortran, in process) runs each reference deck on
 the GPU; its tea.out must reproduce the reference's per-step `step:` rows
 within 1e-10 relative and its iteration counts (tea_solve.f90:448-453)."""
-import os
 
 import pytest
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import tl_oracle as O
-from helpers import assert_close, assert_scalar, interior
+from helpers import assert_scalar
 from tealeaf_ref_amd import _lib as L
 from test_gpu_kernels import Pair, pair_factory  # noqa: F401  (fixture)
 

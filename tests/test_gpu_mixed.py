@@ -24,7 +24,7 @@ import scipy.sparse.linalg as spla
 import tl_oracle as O
 from helpers import interior
 from tealeaf_ref_amd import _lib as L
-from test_gpu_kernels import Pair, pair_factory  # noqa: F401  (fixture)
+from test_gpu_kernels import pair_factory  # noqa: F401  (fixture)
 
 pytestmark = pytest.mark.gpu
 
