@@ -166,7 +166,9 @@ constexpr bool diff_form() {
 #ifndef TL_W_WAVES
 #define TL_W_WAVES 1
 #endif
-// minimum waves per SIMD the two-sweep pass is compiled for
+// minimum waves per SIMD the two-sweep pass is compiled for: FP64 at 4
+// spills 14 registers and runs 210.5 us per pass at 4096^2, against 260.8
+// with 3 waves (no spills) on 1024 tiles and 220.3 on 720
 #ifndef TL_PAIR_WAVES64
 #define TL_PAIR_WAVES64 4
 #endif

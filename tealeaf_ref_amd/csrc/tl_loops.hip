@@ -165,6 +165,9 @@ __device__ inline bool jacobi_pair_exit(int n, double eps_abs, const double *__r
 // Jacobi iterations n and n+1 in one pass (tl_jacobi2.hpp): exit test of the
 // previous pass first; u of iteration n to umid, of n+1 to uout, the
 // partials of sum |u_n - u_n-1| to part_a and of sum |u_n+1 - u_n| to part_b.
+// Compiled for 4 waves per SIMD (5 registers spilled): 153.5 us per pass at
+// 4096^2, against 176.5 with 3 waves (no spills) on 1024 tiles and 170.0 on
+// 720.
 #ifndef TL_JACOBI_PAIR_WAVES
 #define TL_JACOBI_PAIR_WAVES 4
 #endif
