@@ -12,7 +12,11 @@ Prints one JSON line.  `value` is seconds per timestep (lower is better);
 FP64-refined forms at a relative residual of 1e-10 on the inverse-density
 deck;
 `roofline` uses the algorithmic bytes of the CG calc_w kernel (48 B per cell:
-read r, p, Kx, Ky; write p, w) over its HIP-event-timed launches;
+read r, p, Kx, Ky; write p, w) over its HIP-event-timed launches (every 16th
+launch; launches queued past a loop's exit, which return at once, are left
+out) and takes `traffic` from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary; `solvers` times Chebyshev, PPCG and Jacobi steps with
+the same per-kernel rates, on algorithmic and on counter-measured bytes;
 `cpu_baseline` times the reference Fortran/OpenMP build (oracle/_ref) on the
 host cores on a bounded sample of the same deck.
 """
