@@ -3,6 +3,7 @@ point include/tealeaf_hip.h declares; the host library exports its own."""
 import ctypes
 import os
 import re
+import subprocess
 
 from tealeaf_ref_amd import _lib, host
 
@@ -37,6 +38,31 @@ def test_init_rejects_bad_sizes_before_any_device_call():
         assert b"tl_init" in L.tl_last_error()
     n = ctypes.c_long()
     assert L.tl_profile_idle(0, ctypes.byref(n)) == 2  # TL_ERR_STATE
+
+
+def test_header_is_plain_c_and_cxx(tmp_path):
+    """include/tealeaf_hip.h compiles as C99 and as C++ on its own, and the
+    C usage shown for callers compiles against it."""
+    src = tmp_path / "use.c"
+    src.write_text(
+        '#include "tealeaf_hip.h"\n'
+        "int demo(int nx, int ny, const double *rho, const double *e, double rx, double ry) {\n"
+        "  int zb[4] = {1, 1, 1, 1};\n"
+        "  long n; double ms, norm;\n"
+        "  if (tl_init(nx, ny, 1, 0)) return 1;\n"
+        "  tl_set_options(TL_CONDUCTIVITY, TL_PREC_NONE, 0);\n"
+        "  tl_upload(TL_FIELD_DENSITY, rho, 1); tl_upload(TL_FIELD_ENERGY1, e, 1);\n"
+        "  tl_common_init(rx, ry, zb); tl_calc_2norm(0, &norm);\n"
+        "  tl_profile_enable(16); tl_profile_get(0, &n, &ms); tl_profile_idle(0, &n);\n"
+        "  return tl_finalize();\n"
+        "}\n")
+    inc = os.path.join(ROOT, "include")
+    for cmd in (["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I", inc,
+                 str(src)],
+                ["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++",
+                 os.path.join(inc, "tealeaf_hip.h")]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, (cmd, r.stderr)
 
 
 def test_host_library_exports():
