@@ -65,6 +65,37 @@ def test_header_is_plain_c_and_cxx(tmp_path):
         assert r.returncode == 0, (cmd, r.stderr)
 
 
+def header_arity():
+    """Parameter count of every function include/tealeaf_hip.h declares."""
+    text = open(os.path.join(ROOT, "include", "tealeaf_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:int|const char \*)\s*(tl_\w+)\s*\(([^)]*)\)\s*;", text, re.M):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else len(args.split(","))
+    return out
+
+
+def test_bindings_match_the_header():
+    """The Fortran interfaces (tl_hip_api.f90) and the ctypes table (_lib.py)
+    bind every function with the header's parameter count."""
+    want = header_arity()
+    assert len(want) >= 35
+    api = open(os.path.join(ROOT, "tealeaf_ref_amd", "host", "tl_hip_api.f90")).read()
+    bound = re.findall(r"FUNCTION\s+(tl_\w+)\s*\(([^)]*)\)\s*BIND\(C,\s*name=\"(tl_\w+)\"\)", api)
+    assert len(bound) >= 25
+    for fname, args, cname in bound:
+        assert fname == cname and cname in want, cname
+        n = len([a for a in args.split(",") if a.strip()])
+        assert n == want[cname], (cname, n, want[cname])
+    L = _lib.lib()
+    for name, n in want.items():
+        if name == "tl_last_error":
+            continue
+        fn = getattr(L, name)
+        assert fn.argtypes is not None and len(fn.argtypes) == n, (name, fn.argtypes, n)
+
+
 def test_host_library_exports():
     L = host.host_lib()
     for n in ("tea_host_init", "tea_host_step", "tea_host_run", "tea_host_last",
