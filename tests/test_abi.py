@@ -96,6 +96,35 @@ def test_bindings_match_the_header():
         assert fn.argtypes is not None and len(fn.argtypes) == n, (name, fn.argtypes, n)
 
 
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of tl_state, tl_loop_state and tl_mixed_state have
+    the C compiler's size and field offsets."""
+    structs = {"tl_state": _lib.TLState, "tl_loop_state": _lib.LoopState,
+               "tl_mixed_state": _lib.MixedState}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tealeaf_hip.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                               check=True).stdout.splitlines():
+        cname, f, v = line.split()
+        got[(cname, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
 def test_host_library_exports():
     L = host.host_lib()
     for n in ("tea_host_init", "tea_host_step", "tea_host_run", "tea_host_last",
