@@ -5,6 +5,8 @@ import os
 import re
 import subprocess
 
+import pytest
+
 from tealeaf_ref_amd import _lib, host
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -117,6 +119,47 @@ def test_struct_layouts_match_the_header(tmp_path):
     got = {}
     for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                check=True).stdout.splitlines():
+        cname, f, v = line.split()
+        got[(cname, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+FLANG = "/opt/rocm/llvm/bin/amdflang"
+
+
+def test_fortran_struct_layouts_match(tmp_path):
+    """The BIND(C) derived types of tl_hip_api.f90 have the same size and
+    component offsets as the C structs (checked above against ctypes)."""
+    if not os.access(FLANG, os.X_OK):
+        pytest.skip("amdflang not present")
+    structs = {"tl_state": _lib.TLState, "tl_loop_state": _lib.LoopState,
+               "tl_mixed_state": _lib.MixedState}
+    body = ["PROGRAM layout", "  USE ISO_C_BINDING", "  USE tl_hip_api", "  IMPLICIT NONE"]
+    for cname in structs:
+        body.append(f"  TYPE({cname}), TARGET :: v_{cname}")
+    body.append("  INTEGER(c_intptr_t) :: base")
+    for cname, py in structs.items():
+        v = f"v_{cname}"
+        body.append(f"  base = TRANSFER(C_LOC({v}), base)")
+        body.append(f"  PRINT '(A,1X,A,1X,I0)', '{cname}', 'size', C_SIZEOF({v})")
+        for f, _ in py._fields_:
+            body.append(f"  PRINT '(A,1X,A,1X,I0)', '{cname}', '{f}', "
+                        f"TRANSFER(C_LOC({v}%{f}), base) - base")
+    body.append("END PROGRAM layout")
+    (tmp_path / "layout.f90").write_text("\n".join(body) + "\n")
+    api = os.path.join(ROOT, "tealeaf_ref_amd", "host", "tl_hip_api.f90")
+    r = subprocess.run([FLANG, "-c", api, "-o", "api.o"], cwd=tmp_path, capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([FLANG, "layout.f90", "api.o", "-o", "layout"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(tmp_path / "layout")], capture_output=True, text=True, check=True)
+    got = {}
+    for line in out.stdout.splitlines():
         cname, f, v = line.split()
         got[(cname, f)] = int(v)
     for cname, py in structs.items():
