@@ -1,5 +1,8 @@
-"""The C ABI library loads on a machine without a GPU and exports every entry
-point include/tealeaf_hip.h declares; the host library exports its own."""
+"""The C ABI on a machine without a GPU: the library loads and exports every
+entry point include/tealeaf_hip.h declares, the header compiles as C99 and
+C++, the ctypes and Fortran bindings match its parameter counts and struct
+layouts, argument checks that precede any device call hold, and the host
+library exports its own entry points."""
 import ctypes
 import os
 import re
