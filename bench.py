@@ -159,6 +159,34 @@ CLASS_TRAFFIC = {0: ("k_loop_w", "cg_fp64_4096_hbm_traffic.md"),
                  6: ("k_pp_ur", "ppcg_fp64_4096_hbm_traffic.md")}
 
 
+def work_launches(solver, rows, inner, max_iters=10000):
+    """Launches per kernel class that did work in the steps of `rows`
+    (TeaHost.last()), from the iteration counts: CG iterations before the
+    switch (two passes each), then Chebyshev iterations two per pass, PPCG
+    outer iterations (a w pass, an update pass and ceil(inner/2) sweep passes
+    each) or Jacobi iterations two per pass.  The printed count is n-1 at an
+    exit at iteration n (tea_solve.f90:448), max_iters at the cap."""
+    n = dict.fromkeys(range(7), 0)
+    for r in rows:
+        its = int(r["iterations"])
+        end = its if its >= max_iters else its + 1  # last executed iteration
+        sw = int(r["switch_n"])
+        cg = end if (solver in ("cg", "jacobi") or sw <= 0) else sw - 1
+        if solver == "jacobi":
+            n[4] += (end + 1) // 2
+            continue
+        n[0] += cg
+        n[1] += cg
+        rest = end - cg
+        if solver == "chebyshev":
+            n[3] += max(0, rest - 1) // 2   # the first Chebyshev step runs per routine
+        elif solver == "ppcg":
+            n[5] += rest
+            n[6] += rest
+            n[2] += rest * ((inner + 1) // 2)
+    return n
+
+
 def solver_runs(cells, steps, warmup):
     """Chebyshev (20 CG presteps), PPCG (10 inner steps) and Jacobi on the
     same 4096^2 deck at tl_eps 1e-15, tl_max_iters 10000 (Chebyshev and Jacobi
@@ -194,36 +222,43 @@ def solver_runs(cells, steps, warmup):
                     k["traffic_source"] = "profiles/" + prof_file
                 kern[name] = k
         rec["kernels"] = kern
+        # the step's rate in counter-measured bytes: launches that did work x
+        # the measured bytes per launch of their class, over the step time;
+        # `covered` is the share of the step those launches take
+        counts = work_launches(solver, prof["rows"], kw.get("inner_steps", 0))
+        mbytes = covered_ms = 0.0
+        for cls, (name, _) in CLASS_BYTES.items():
+            k = kern.get(name)
+            if counts[cls] and k and "measured_bytes_per_launch" in k:
+                mbytes += counts[cls] * k["measured_bytes_per_launch"]
+                covered_ms += counts[cls] * k["avg_launch_ms"]
+        if mbytes:
+            rec["step_measured_gbs"] = mbytes / elapsed / 1e9
+            rec["step_measured_frac"] = rec["step_measured_gbs"] / PEAK_HBM_GBS
+            rec["step_time_covered"] = covered_ms * 1e-3 / elapsed
+            rec["work_launches"] = {CLASS_BYTES[c][0]: v for c, v in counts.items() if v}
         log(f"[bench] {cells}^2 {solver}: {rec}")
         out[solver] = rec
     return out
 
 
-def stream_bandwidth(n=1 << 28, reps=20):
-    """Achievable HBM bandwidth on this GPU, STREAM style, with torch's
-    elementwise kernels on FP64 arrays of n elements (2 GiB each, far beyond
-    the 256 MB Infinity Cache): copy (16 B per element) and triad
-    c = a + s b (24 B per element), timed with events over `reps` launches."""
-    import torch
-    dev = torch.device("cuda", 0)
-    a = torch.rand(n, dtype=torch.float64, device=dev)
-    b = torch.rand(n, dtype=torch.float64, device=dev)
-    c = torch.empty_like(a)
-    out = {"elements": n, "dtype": "f64"}
-    for name, fn, nbytes in (("copy", lambda: c.copy_(a), 16.0),
-                             ("triad", lambda: torch.add(a, b, alpha=3.0, out=c), 24.0)):
-        fn()
-        torch.cuda.synchronize()
-        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0.record()
-        for _ in range(reps):
-            fn()
-        t1.record()
-        torch.cuda.synchronize()
-        ms = t0.elapsed_time(t1) / reps
-        out[name + "_gbs"] = nbytes * n / (ms * 1e-3) / 1e9
-    del a, b, c
-    torch.cuda.empty_cache()
+def stream_bandwidth(n=1 << 28, reps=10):
+    """Achievable HBM bandwidth on this GPU, STREAM style, from the library's
+    hand-written copy (16 B per element) and triad (24 B per element) kernels
+    on three FP64 arrays of n elements (2 GiB each, far beyond the 256 MB
+    Infinity Cache): the best over plain and streaming accesses and grid
+    sizes (tl_hbm_stream)."""
+    import ctypes as C
+    from tealeaf_ref_amd import _lib
+    d = _lib.Device(64, 64)
+    try:
+        cp, tr = C.c_double(), C.c_double()
+        _lib.call("tl_hbm_stream", n, reps, C.byref(cp), C.byref(tr))
+    finally:
+        d.close()
+    out = {"elements": n, "dtype": "f64", "kernels": "hand-written HIP (tl_hbm_stream)",
+           "copy_gbs": cp.value, "triad_gbs": tr.value,
+           "copy_frac": cp.value / PEAK_HBM_GBS, "triad_frac": tr.value / PEAK_HBM_GBS}
     log(f"[bench] stream bandwidth: {out}")
     return out
 
@@ -243,12 +278,13 @@ def small_run(steps=10):
     return rec
 
 
-def run_deck_timed(deck, steps, warmup, workdir, want_u=False, prof_out=None):
+def run_deck_timed(deck, steps, warmup, workdir, want_u=False, prof_out=None, residual=None):
     """Run `warmup` + `steps` timesteps of a deck through the native host;
     returns the timed seconds, the per-step results of the timed steps and
     the final u (interior) when asked.  With a dict `prof_out`, every
     EVENT_PERIOD-th launch of each kernel class is timed and prof_out[class]
-    = (timed launches, their milliseconds, idle launches)."""
+    = (timed launches, their milliseconds, idle launches).  With residual =
+    (rx, ry), also returns host_true_residual of the last step."""
     import ctypes as C
     import numpy as np
     from tealeaf_ref_amd import _lib
@@ -284,6 +320,11 @@ def run_deck_timed(deck, steps, warmup, workdir, want_u=False, prof_out=None):
         u = np.zeros((n + 2, n + 2))
         _lib.call("tl_download", _lib.FIELD["u"], u.ctypes.data_as(C.POINTER(C.c_double)), 1)
         u = u[1:-1, 1:-1].copy()
+    if residual is not None:
+        m = re.search(r"x_cells=(\d+)", deck)
+        tres = host_true_residual(int(m.group(1)), *residual)
+        h.close()
+        return elapsed, rows, u, tres
     h.close()
     return elapsed, rows, u
 
@@ -318,40 +359,81 @@ def mixed_kernel_rates(prof, cells):
     return out
 
 
-def precision_comparison(cells, steps, warmup, inner_steps=10):
+def host_true_residual(cells, rx, ry):
+    """||u0 - A u|| / ||u0 - A u0|| of the step the device holds (u0 its
+    right-hand side, u its solution), evaluated on the host from the
+    downloaded fields: A in difference form in FP64 (rounding relative to the
+    fluxes, not to the diagonal), the same evaluation for every form."""
+    import numpy as np
+    from tealeaf_ref_amd import _lib
+    u = _lib.download_field("u", cells, cells)
+    u0 = _lib.download_field("u0", cells, cells)[1:-1, 1:-1].copy()
+    kx = _lib.download_field("kx", cells, cells)
+    ky = _lib.download_field("ky", cells, cells)
+    kl, kr = kx[1:-1, 1:-1], kx[1:-1, 2:]   # faces left / right of each cell
+    kd, ku = ky[1:-1, 1:-1], ky[2:, 1:-1]   # below / above
+
+    def resid_norm(x):
+        c = x[1:-1, 1:-1]
+        a = c + rx * (kl * (c - x[1:-1, :-2]) + kr * (c - x[1:-1, 2:])) \
+            + ry * (kd * (c - x[:-2, 1:-1]) + ku * (c - x[2:, 1:-1]))
+        return float(np.sqrt(np.sum((u0 - a) ** 2)))
+
+    res = resid_norm(u)
+    x0 = np.zeros_like(u)
+    x0[1:-1, 1:-1] = u0
+    res0 = resid_norm(x0)
+    return res / res0 if res0 > 0.0 else 0.0
+
+
+# tl_eps of the FP64 forms in the precision comparison: the largest of 1e-10,
+# 1e-11, ... at which their exit (the reference's recurrence test) leaves a
+# true relative residual <= 1e-10 (measured on this deck at 4096^2: CG's
+# recurrence residual tracks the true one, 9.99e-11 at 1e-10; PPCG's exit
+# quantity (r - r_store).z does not: 1.5e-8 at 1e-10, 1.5e-10 at 1e-12,
+# 1.5e-11 at 1e-13); the mixed forms stop on the true residual itself
+PRECISION_EPS = {"fp64_cg": 1e-10, "fp64_ppcg": 1e-13, "mixed_cg": 1e-10, "mixed_ppcg": 1e-10}
+
+
+def precision_comparison(cells, steps, warmup, inner_steps=10, target=1e-10):
     """FP64 CG, FP64 PPCG and the FP32-inner / FP64-refined CG and PPCG on the
-    inverse-density deck, each to a relative residual of 1e-10 (FP64: the
-    reference's recurrence test plus tl_check_result for the true residual;
-    mixed: the true residual).  Errors of u and of the summary are relative
-    to the FP64 CG result."""
+    inverse-density deck, each to a true relative residual of `target`.  The
+    FP64 forms stop on the reference's recurrence test (tea_solve.f90:410), at
+    the tl_eps of PRECISION_EPS that brings their true residual to the target;
+    the mixed forms stop on the true residual.  Every form's result is then
+    measured the same way (host_true_residual).  Errors of u and of the
+    summary are relative to the FP64 CG result."""
     from tealeaf_ref_amd.deckgen import bench_deck
     import numpy as np
     runs = {}
-    base = dict(coefficient="inverse_density", eps=1e-10, max_iters=100000,
-                end_step=steps + warmup, summary_frequency=1, check_result=True)
+    base = dict(coefficient="inverse_density", max_iters=100000, end_step=steps + warmup,
+                summary_frequency=1)
     plan = [("fp64_cg", "cg", ()), ("fp64_ppcg", "ppcg", ()),
             ("mixed_cg", "cg", ("tl_mixed_precision",)),
             ("mixed_ppcg", "ppcg", ("tl_mixed_precision",))]
     ref_u = ref_sum = None
+    dx = 10.0 / cells
+    rx = 0.004 / (dx * dx)
     for name, solver, extra in plan:
-        kw = dict(base, extra=extra)
+        kw = dict(base, extra=extra, eps=PRECISION_EPS[name])
         if solver == "ppcg":
             kw["inner_steps"] = inner_steps
         deck = bench_deck(cells, solver=solver, **kw)
         prof = {} if name == "mixed_ppcg" else None
         with tempfile.TemporaryDirectory() as wd:
-            elapsed, rows, u = run_deck_timed(deck, steps, warmup, wd, want_u=True,
-                                              prof_out=prof)
+            elapsed, rows, u, tres = run_deck_timed(deck, steps, warmup, wd, want_u=True,
+                                                    prof_out=prof, residual=(rx, rx))
         last = rows[-1]
-        rec = {"s_per_step": elapsed / steps,
+        rec = {"tl_eps": PRECISION_EPS[name], "s_per_step": elapsed / steps,
                "iterations_per_step": sum(r["iterations"] for r in rows) / steps,
-               "true_rel_residual": max(
-                   r["exact_error"] if not extra else r["conduction_error"] for r in rows)}
+               "true_rel_residual": tres, "meets_target": tres <= target}
         if solver == "ppcg":
             rec["sweeps_per_step"] = sum(r["ppcg_inner"] for r in rows) / steps
         if extra:
             rec["refinements_per_step"] = sum(r["refinements"] for r in rows) / steps
+            rec["solver_true_rel_residual"] = max(r["conduction_error"] for r in rows)
         else:
+            rec["recurrence_rel_residual"] = max(r["conduction_error"] for r in rows)
             # HBM rate in the algorithmic bytes of the reference's dataflow
             rec["model_gbs"] = (model_bytes_per_cell(solver, rows) * cells * cells / elapsed
                                 / 1e9)
@@ -367,14 +449,16 @@ def precision_comparison(cells, steps, warmup, inner_steps=10):
         log(f"[bench] {cells}^2 {name}: {rec}")
         runs[name] = rec
     out = {"workload": f"{cells}x{cells}, seeded deck (3 rectangles + 2 circles, seed 2), "
-                       f"tl_coefficient_inverse_density, relative residual 1e-10, "
+                       f"tl_coefficient_inverse_density, true relative residual {target:g}, "
                        f"PPCG {inner_steps} inner steps, {steps} timed steps after {warmup}",
-           "runs": runs}
-    fp64 = runs["fp64_cg"]["s_per_step"]
-    out["speedup_mixed_ppcg_vs_fp64_cg"] = fp64 / runs["mixed_ppcg"]["s_per_step"]
-    out["speedup_mixed_cg_vs_fp64_cg"] = fp64 / runs["mixed_cg"]["s_per_step"]
-    out["speedup_mixed_ppcg_vs_fp64_ppcg"] = (runs["fp64_ppcg"]["s_per_step"]
-                                              / runs["mixed_ppcg"]["s_per_step"])
+           "runs": runs,
+           "all_meet_target": all(r["meets_target"] for r in runs.values())}
+    t = {k: v["s_per_step"] for k, v in runs.items()}
+    out["speedup_mixed_ppcg_vs_fp64_cg"] = t["fp64_cg"] / t["mixed_ppcg"]
+    out["speedup_mixed_cg_vs_fp64_cg"] = t["fp64_cg"] / t["mixed_cg"]
+    out["speedup_mixed_ppcg_vs_fp64_ppcg"] = t["fp64_ppcg"] / t["mixed_ppcg"]
+    out["speedup_best_mixed_vs_best_fp64"] = min(t["fp64_cg"], t["fp64_ppcg"]) / min(
+        t["mixed_cg"], t["mixed_ppcg"])
     return out
 
 
@@ -401,14 +485,20 @@ def deck_fields(cells, coefficient="density"):
 
 
 def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed=5,
-                  solver="cg", save=None, compare=None):
+                  solver="cg", save=None, compare=None, fd_eps=1e-12):
     """The adjoint case: gradient of J = sum over a seeded 256x256 window of
     (u_N - u_target)^2 with respect to per-cell log-density, u_target from a
     forward on a log-normally perturbed density (sigma 0.1), computed once
     (by the first precision) and shared; timed forward and backward
     (checkpoint every `every` steps) with CG or PPCG solves (PPCG: 10 inner
-    sweeps, bounds from 30 CG iterations of each solve); central differences
-    with step h on n_fd seeded cells inside the window.
+    sweeps, bounds from 30 CG iterations of each solve).
+
+    Finite differences: central differences with step h on n_fd seeded cells
+    inside the window.  The perturbed forwards run the refined solver (FP32
+    inner CG, FP64 refinement to a true relative residual of fd_eps), so the
+    difference quotient resolves the gradient rather than the forward
+    solver's own stopping error; the same quotients are compared with the
+    gradient of every precision.
 
     A case too long for one run is split in two: `save` writes the target on
     the window, the gradient on the window and on every 8th row and column,
@@ -454,7 +544,8 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
         t0 = time.perf_counter()
         u = p.forward(theta)
         t_f = time.perf_counter() - t0
-        its_f, sw_f, unc_f = p.iterations, p.sweeps, p.unconverged
+        its_f, sw_f = p.iterations, p.sweeps
+        stops_f = (p.maxiter_stops, p.floor_stops)
         seed_g = np.zeros_like(u)
         seed_g[win] = 2.0 * (u[win] - target[win])
         J = float(((u[win] - target[win]) ** 2).sum())
@@ -470,24 +561,10 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
                "backward_iterations_per_step": its_b / steps,
                "forward_sweeps_per_step": sw_f / steps,
                "backward_sweeps_per_step": p.sweeps / steps,
-               "eps": p.eps, "unconverged_solves": unc_f + p.unconverged,
+               "eps": p.eps, "solves": 2 * steps,
+               "unconverged_solves": {"max_iters": stops_f[0] + p.maxiter_stops,
+                                      "rounding_floor": stops_f[1] + p.floor_stops},
                "device_bytes": nbytes.value}
-        fd_rng = np.random.default_rng(seed + 1)
-        errs = []
-        for _ in range(n_fd):
-            c = (int(fd_rng.integers(wy, wy + 256)), int(fd_rng.integers(wx, wx + 256)))
-            tp, tm = theta.copy(), theta.copy()
-            tp[c] += h
-            tm[c] -= h
-            up, um = p.forward(tp), p.forward(tm)
-            fd = (float(((up[win] - target[win]) ** 2).sum())
-                  - float(((um[win] - target[win]) ** 2).sum())) / (2 * h)
-            errs.append(abs(fd - g[c]) / abs(g[c]))
-            log(f"[bench] adjoint {prec} FD cell {c}: fd {fd:.10e} adjoint {g[c]:.10e} "
-                f"rel {errs[-1]:.2e}")
-        if errs:
-            rec.update({"fd_cells": n_fd, "fd_h": h, "fd_max_rel_err": max(errs),
-                        "fd_median_rel_err": float(np.median(errs))})
         if save:
             np.savez(save, target_window=target[win], g_window=g[win], g_sub=g[::8, ::8],
                      g_norm=np.linalg.norm(g), J=J, precision=prec, solver=solver)
@@ -506,61 +583,138 @@ def adjoint_bench(cells, steps, every, n_fd, h, precision, mixed_eps=1e-12, seed
     if len(grads) == 2:
         out["mixed_vs_fp64_gradient_rel_l2"] = float(
             np.linalg.norm(grads["mixed"] - grads["fp64"]) / np.linalg.norm(grads["fp64"]))
+    if n_fd:
+        # central differences with the refined solver, against every gradient
+        p = TeaLeafSteps(e0, rx, ry, steps, coefficient=_lib.CONDUCTIVITY, eps=fd_eps,
+                         checkpoint_every=every, precision="mixed", solver="cg", max_iters=20000)
+        fd_rng = np.random.default_rng(seed + 1)
+        cells_fd, fds = [], []
+        t0 = time.perf_counter()
+        for _ in range(n_fd):
+            c = (int(fd_rng.integers(wy, wy + 256)), int(fd_rng.integers(wx, wx + 256)))
+            tp, tm = theta.copy(), theta.copy()
+            tp[c] += h
+            tm[c] -= h
+            up, um = p.forward(tp), p.forward(tm)
+            fd = (float(((up[win] - target[win]) ** 2).sum())
+                  - float(((um[win] - target[win]) ** 2).sum())) / (2 * h)
+            cells_fd.append(c)
+            fds.append(fd)
+            log(f"[bench] adjoint FD cell {c}: fd {fd:.10e} " + " ".join(
+                f"{k} {grads[k][c]:.10e}" for k in grads))
+        fd_s = time.perf_counter() - t0
+        p.close()
+        out["finite_differences"] = {
+            "cells": n_fd, "h": h, "forward_solver": f"mixed precision CG, true relative "
+                                                    f"residual {fd_eps:g}",
+            "seconds": fd_s}
+        for k, g in grads.items():
+            errs = [abs(fd - g[c]) / abs(g[c]) for c, fd in zip(cells_fd, fds)]
+            out[k].update({"fd_cells": n_fd, "fd_h": h, "fd_max_rel_err": max(errs),
+                           "fd_median_rel_err": float(np.median(errs))})
+            log(f"[bench] adjoint {k} vs FD: max {max(errs):.2e} median {np.median(errs):.2e}")
     return out
 
 
-def cpu_baseline(cells, solver, gpu_iters_per_step, cap):
+def cpu_info():
+    """The host's processors: logical CPUs, the CPUs this process may run on,
+    and from lscpu the model, sockets, cores per socket and threads per core."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        text = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+    except (OSError, subprocess.SubprocessError):
+        text = ""
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core"}
+    for line in text.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    if isinstance(info.get("sockets"), int) and isinstance(info.get("cores_per_socket"), int):
+        info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    return info
+
+
+def _reference_time_per_it(ref_bin, deck, threads):
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "tea.in"), "w") as f:
+            f.write(deck)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        t0 = time.time()
+        subprocess.run([ref_bin], cwd=d, env=env, check=True, timeout=900,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        wall = time.time() - t0
+        text = open(os.path.join(d, "tea.out")).read()
+    m = re.search(r"Solve Time\s+(\S+)\s+Its\s+(\d+)\s+Time Per It\s+(\S+)", text)
+    return float(m.group(3)), int(m.group(2)), wall
+
+
+def cpu_baseline(cells, solver, gpu_iters_per_step, cap, cap_one_thread):
     """Reference build (oracle/_ref) on the host cores: one timestep capped at
-    `cap` iterations, profiler_on -> Time Per It; falls back to the C oracle
-    (single thread) when the reference build is absent."""
+    `cap` iterations with profiler_on -> "Time Per It" (tea_solve.f90:474),
+    with OpenMP threads = the physical cores this process may use, and once
+    more on one thread (capped at `cap_one_thread`); falls back to the C
+    oracle (single thread) when the reference build is absent."""
     from tealeaf_ref_amd.deckgen import bench_deck
-    deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=cap, end_step=1,
-                      summary_frequency=1, profiler=True)
+    info = cpu_info()
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "tea_leaf")
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # the physical cores, within this process's CPU share (the CPUs it may run
+    # on, and the OpenMP thread count the environment sets for a GPU job)
+    limits = [info["affinity_cpus"], info.get("physical_cores") or info["affinity_cpus"]]
+    if (info["omp_num_threads_env"] or "").isdigit():
+        limits.append(int(info["omp_num_threads_env"]))
+    threads = max(1, min(limits))
     if os.path.exists(ref_bin):
-        with tempfile.TemporaryDirectory() as d:
-            with open(os.path.join(d, "tea.in"), "w") as f:
-                f.write(deck)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-            t0 = time.time()
-            subprocess.run([ref_bin], cwd=d, env=env, check=True, timeout=600,
-                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-            wall = time.time() - t0
-            text = open(os.path.join(d, "tea.out")).read()
-        m = re.search(r"Solve Time\s+(\S+)\s+Its\s+(\d+)\s+Time Per It\s+(\S+)", text)
-        tpi = float(m.group(3))
+        deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=cap, end_step=1,
+                          summary_frequency=1, profiler=True)
+        tpi, its, wall = _reference_time_per_it(ref_bin, deck, threads)
+        deck1 = bench_deck(cells, solver=solver, eps=1e-15, max_iters=cap_one_thread, end_step=1,
+                           summary_frequency=1, profiler=True)
+        tpi1, its1, wall1 = _reference_time_per_it(ref_bin, deck1, 1)
         kind, cores = "reference", threads
         sample = (f"{cells}^2 FP64 {solver.upper()}, 1 timestep capped at {cap} iterations, "
                   f"OMP_NUM_THREADS={threads} (reference Fortran/OpenMP built from source by "
                   f"oracle/ref.mk, {wall:.1f} s wall); s/timestep = Time Per It x the GPU run's "
                   f"mean iterations per step")
+        one = {"threads": 1, "time_per_iteration_s": tpi1, "iterations": its1,
+               "s_per_timestep": tpi1 * gpu_iters_per_step, "wall_s": wall1,
+               "cell_iterations_per_s": cells * cells / tpi1,
+               "sample": f"the same deck capped at {cap_one_thread} iterations, one thread"}
     else:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import tl_oracle as O
+        deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=cap_one_thread, end_step=1,
+                          summary_frequency=1, profiler=True)
         p = O.parse_deck(deck)
         t0 = time.perf_counter()
         rec = O.run(p)
         wall = time.perf_counter() - t0
         tpi = wall / max(1, rec[-1]["iterations"] + 1)
         kind, cores = "port", 1
-        sample = f"C oracle, {cells}^2 FP64 {solver.upper()}, 1 step capped at {cap} iterations"
+        sample = (f"C oracle, {cells}^2 FP64 {solver.upper()}, 1 step capped at {cap_one_thread} "
+                  f"iterations")
+        one = {"threads": 1, "time_per_iteration_s": tpi, "s_per_timestep": tpi * gpu_iters_per_step}
     s_per_step = tpi * gpu_iters_per_step
-    return {"value": s_per_step, "unit": "s/timestep", "cores": cores, "kind": kind,
-            "sample": sample, "time_per_iteration_s": tpi,
-            "cell_iterations_per_s": cells * cells / tpi}
+    return {"value": s_per_step, "unit": "s/timestep", "cores": cores, "threads": cores,
+            "kind": kind, "sample": sample, "time_per_iteration_s": tpi,
+            "cell_iterations_per_s": cells * cells / tpi,
+            "physical_cores": info.get("physical_cores"), "host": info, "one_thread": one}
 
 
-def profile_run(cells, iters, copies=4):
-    """Short run for hardware-counter profiles: one 4096^2 CG timestep capped
-    at `iters` iterations, then `copies` launches of the plain copy kernel
-    (tl_ppcg_store_r: r_store = r over the interior, 8 B read + 8 B written
-    per cell), whose counters calibrate FETCH_SIZE / WRITE_SIZE."""
+def profile_run(cells, iters, solver="cg", copies=4):
+    """Short run for hardware-counter profiles: one timestep of the seeded
+    deck (CG, or PPCG with 10 inner steps) capped at `iters` iterations, then
+    `copies` launches of the plain copy kernel (tl_ppcg_store_r: r_store = r
+    over the interior, 8 B read + 8 B written per cell), whose counters
+    calibrate FETCH_SIZE / WRITE_SIZE."""
     from tealeaf_ref_amd import _lib
     from tealeaf_ref_amd.deckgen import bench_deck
     from tealeaf_ref_amd.host import TeaHost
-    deck = bench_deck(cells, solver="cg", eps=1e-15, max_iters=iters, end_step=1,
-                      summary_frequency=1)
+    kw = {"inner_steps": 10} if solver == "ppcg" else {}
+    deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=iters, end_step=1,
+                      summary_frequency=1, **kw)
     with tempfile.TemporaryDirectory() as wd:
         path = os.path.join(wd, "tea.in")
         with open(path, "w") as f:
@@ -572,8 +726,9 @@ def profile_run(cells, iters, copies=4):
         _lib.call("tl_synchronize")
         it = h.last()["iterations"]
         h.close()
-    print(json.dumps({"profile_run": True, "cells": cells * cells, "cg_iterations": it,
-                      "copy_launches": copies, "copy_bytes_each": 16 * cells * cells}))
+    print(json.dumps({"profile_run": True, "solver": solver, "cells": cells * cells,
+                      "iterations": it, "copy_launches": copies,
+                      "copy_bytes_each": 16 * cells * cells}))
 
 
 def main():
@@ -584,10 +739,14 @@ def main():
     ap.add_argument("--cells", type=int, default=4096)
     ap.add_argument("--solver", default="cg", choices=["cg"])
     ap.add_argument("--cpu-cap", type=int, default=1500, help="iterations of the CPU sample")
+    ap.add_argument("--cpu-cap-one-thread", type=int, default=100,
+                    help="iterations of the one-thread CPU sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mixed-sizes", default="4096",
+    ap.add_argument("--mixed-sizes", default="4096,8192",
                     help="comma-separated mesh sizes of the FP64 vs FP32-refined comparison "
                          "('' to skip)")
+    ap.add_argument("--profile-solver", default="cg", choices=["cg", "ppcg"],
+                    help="solver of the counter-profiling mode")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="counter-profiling mode: one CG step capped at this many iterations "
                          "plus copy-kernel calibration launches; prints a short JSON line")
@@ -595,7 +754,7 @@ def main():
                     help="mesh of the adjoint case (0 to skip)")
     ap.add_argument("--adjoint-steps", type=int, default=20)
     ap.add_argument("--adjoint-every", type=int, default=5, help="checkpoint interval")
-    ap.add_argument("--adjoint-fd", type=int, default=4, help="finite-difference cells")
+    ap.add_argument("--adjoint-fd", type=int, default=32, help="finite-difference cells")
     ap.add_argument("--adjoint-h", type=float, default=1e-6)
     ap.add_argument("--adjoint-precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--adjoint-solver", default="cg", choices=["cg", "ppcg"])
@@ -606,6 +765,10 @@ def main():
                          "and compare the gradient with the one saved there")
     ap.add_argument("--adjoint-mixed-eps", type=float, default=1e-12,
                     help="true relative residual of the mixed-precision solves")
+    ap.add_argument("--adjoint-large-cells", type=int, default=8192,
+                    help="mesh of the bounded mixed + adjoint ride-along (0 to skip)")
+    ap.add_argument("--adjoint-large-steps", type=int, default=2,
+                    help="timesteps of the bounded mixed + adjoint ride-along")
     ap.add_argument("--no-small", action="store_true", help="skip the 512^2 CG case")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip the STREAM-style copy/triad bandwidth measurement")
@@ -620,7 +783,7 @@ def main():
         sys.exit(2)
 
     if args.profile_iters:
-        profile_run(args.cells, args.profile_iters)
+        profile_run(args.cells, args.profile_iters, args.profile_solver)
         return
 
     with tempfile.TemporaryDirectory() as wd:
@@ -681,15 +844,26 @@ def main():
     sizes = [int(x) for x in args.mixed_sizes.split(",") if x.strip()]
     if sizes:
         out["precision_comparison"] = [
-            precision_comparison(n, args.mixed_steps, args.mixed_warmup) for n in sizes]
+            precision_comparison(n, *((args.mixed_steps, args.mixed_warmup) if n < 8192
+                                      else (1, 0)))  # 8192^2: the first step, once
+            for n in sizes]
     if args.adjoint_cells:
         out["adjoint"] = adjoint_bench(args.adjoint_cells, args.adjoint_steps, args.adjoint_every,
                                        args.adjoint_fd, args.adjoint_h, args.adjoint_precision,
                                        args.adjoint_mixed_eps, solver=args.adjoint_solver,
                                        save=args.adjoint_save, compare=args.adjoint_compare)
+    if args.adjoint_large_cells and args.adjoint_large_steps:
+        # the mixed-precision + adjoint case at full size, bounded: FP64 and
+        # mixed CG forward and backward over a few steps, checkpoint every
+        # step (the 50-step case runs as two invocations, --adjoint-save then
+        # --adjoint-compare)
+        out["adjoint_large"] = adjoint_bench(
+            args.adjoint_large_cells, args.adjoint_large_steps, 1, 0, args.adjoint_h, "both",
+            args.adjoint_mixed_eps, solver="cg")
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap)
+            out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap,
+                                               args.cpu_cap_one_thread)
         except Exception as e:  # report, do not fail the GPU measurement
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out))

@@ -203,6 +203,15 @@ typedef struct {
 #define TL_LOOP_CONVERGED 1  /* iteration n_end met the exit test            */
 #define TL_LOOP_SWITCH 2     /* switch condition true at the top of n_end    */
 #define TL_LOOP_MAXITERS 3   /* n_end == n_last without meeting the test     */
+#define TL_LOOP_NONFINITE 4  /* a reduction (p.w, r.r, z.r or the Jacobi
+                              * error) of iteration n_end was NaN or Inf; the
+                              * call returns TL_ERR_NONFINITE.  The reference
+                              * does not check (it iterates to max_iters); a
+                              * zero density gives such values through Kx/Ky
+                              * (kernels/tea_leaf_common_kernel.f90:97-98). */
+#define TL_LOOP_FLOOR 5      /* tl_mixed_solve only: two refinements in a row
+                              * failed to halve the true residual, which has
+                              * reached its rounding floor above rel_tol     */
 
 /* CG branch (tea_solve.f90:341-369).  cg_alphas/cg_betas are host arrays
  * indexed by n (entry n written for every executed iteration), as
@@ -218,6 +227,13 @@ int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const double *betas,
 
 /* Jacobi branch (tea_solve.f90:375-379). */
 int tl_jacobi_loop(tl_loop_state *st);
+
+/* The exit-test error of every iteration lo..hi the last tl_cg_loop,
+ * tl_ppcg_loop or tl_jacobi_loop call executed: err[n] = sqrt|rrn| (CG, PPCG)
+ * or sqrt|error| (Jacobi) of iteration n, the value the reference prints as
+ * "Residual" after each iteration with verbose_on (tea_solve.f90:402-408).
+ * err is a host array indexed by n, as cg_alphas. */
+int tl_loop_errors(int lo, int hi, double *err);
 
 /* Chebyshev iterations n_first..n_last with coefficient index
  * step_first.. (tea_solve.f90:265-279), queued without synchronisation;
@@ -237,7 +253,7 @@ int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int step_first
  * residual of the starting u (a true residual, unlike the recurrence
  * residual of tea_solve.f90:402-410), after max_outer refinements, or when
  * two refinements in a row fail to halve the true residual (it has reached
- * its rounding floor; reason TL_LOOP_MAXITERS).  Requires
+ * its rounding floor; reason TL_LOOP_FLOOR).  Requires
  * tl_preconditioner_type none and a chunk with all four faces external. */
 #define TL_MIXED_CG 1
 #define TL_MIXED_PPCG 2
@@ -260,7 +276,7 @@ typedef struct {
   int outer;              /* refinements done                                    */
   int inner_iters;        /* CG or PPCG outer iterations summed over inner solves */
   int sweeps;             /* PPCG polynomial sweeps summed                       */
-  int reason;             /* TL_LOOP_CONVERGED or TL_LOOP_MAXITERS               */
+  int reason;             /* TL_LOOP_CONVERGED, TL_LOOP_FLOOR or TL_LOOP_MAXITERS */
   double initial_residual;/* ||u0 - A u|| at entry                               */
   double final_residual;  /* ||u0 - A u|| on exit (FP64)                         */
 } tl_mixed_state;
@@ -298,6 +314,14 @@ int tl_adjoint_face_grad(int lambda_field, int u_slot, int e0_slot, int grad_slo
 /* Device memory the library holds (fields, FP32 fields, snapshot slots,
  * reduction buffers), in bytes.  No counterpart in the reference. */
 int tl_device_bytes(double *bytes);
+
+/* Achievable HBM bandwidth of the device, STREAM style: the best copy
+ * (c = a, 16 B per element) and triad (c = a + s b, 24 B per element) rate
+ * in GB/s of hand-written kernels over three FP64 arrays of n elements (n
+ * even; allocated for the call and freed), each timed over `reps` launches.
+ * No counterpart in the reference: the ceiling the solver kernels' rates are
+ * compared with.  Synchronises. */
+int tl_hbm_stream(long n, int reps, double *copy_gbs, double *triad_gbs);
 
 /* ---- timing of the dominant kernels (HIP events on the library stream) --- */
 /* on = 1 times every launch of the solver kernels with HIP events, on = N > 1

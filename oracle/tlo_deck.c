@@ -1,8 +1,9 @@
 /* Runs tea.in decks through the oracle (tlo_run) and prints, per deck, the
  * records of every timestep: step, iteration count and the field summary.
- * Checker tooling: built with the sanitizers by `make -f oracle/Makefile
- * sanitized` to run the oracle's C under AddressSanitizer and
- * UndefinedBehaviorSanitizer; never part of the product. */
+ * Checker tooling: built with the sanitizers by `make -f oracle/sanitize.mk`
+ * to run the oracle's C under AddressSanitizer and UndefinedBehaviorSanitizer;
+ * never part of the product.  Exits 1 if any deck could not be read, parsed
+ * or run, so a sweep whose decks stop parsing does not look clean. */
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -44,11 +45,13 @@ int main(int argc, char **argv) {
     if (tlo_parse_deck(text, &p, msg, (int)sizeof msg) != 0) {
       printf("%s: %s\n", argv[a], msg);
       free(text);
+      status = 1;
       continue;
     }
     const int n = tlo_run(&p, rec, MAX_RECORDS, 0, msg, (int)sizeof msg);
     if (n < 0) {
       printf("%s: %s\n", argv[a], msg);
+      status = 1;
     } else {
       for (int i = 0; i < n; ++i)
         printf("%s step %d iterations %d temperature %.17g\n", argv[a], rec[i].step,

@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(_PKG, "libtealeaf_hip.so")
 TL_OK = 0
 CONDUCTIVITY, RECIP_CONDUCTIVITY = 1, 2
 PREC_NONE, PREC_JAC_DIAG, PREC_JAC_BLOCK = 1, 2, 3
-LOOP_CONVERGED, LOOP_SWITCH, LOOP_MAXITERS = 1, 2, 3
+LOOP_CONVERGED, LOOP_SWITCH, LOOP_MAXITERS, LOOP_NONFINITE, LOOP_FLOOR = 1, 2, 3, 4, 5
+ERR_HIP, ERR_STATE, ERR_ARG, ERR_NONFINITE, ERR_EIGEN = 1, 2, 3, 4, 5
 
 FIELD = {"density": 1, "energy0": 2, "energy1": 3, "u": 4, "p": 5, "sd": 6, "r": 7, "z": 8,
          "kx": 9, "ky": 10, "di": 11, "u0": 12, "w": 13, "r_store": 14, "mi": 15,
@@ -80,7 +81,7 @@ def lib():
             "tl_ppcg_update_z": [], "tl_jacobi_solve": [PD],
             "tl_cg_loop": [P(LoopState), PD, PD],
             "tl_ppcg_loop": [P(LoopState), PD, PD, I, D, P(I)],
-            "tl_jacobi_loop": [P(LoopState)],
+            "tl_jacobi_loop": [P(LoopState)], "tl_loop_errors": [I, I, PD],
             "tl_cheby_run": [PD, PD, I, I, I, PD],
             "tl_mixed_solve": [P(MixedState)],
             "tl_generate": [D, D, D, D, I, P(TLState)],
@@ -89,7 +90,7 @@ def lib():
             "tl_snapshot_download": [I, PD, I], "tl_adjoint_face_grad": [I, I, I, I],
             "tl_device_bytes": [PD], "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
             "tl_profile_idle": [I, P(C.c_long)],
-            "tl_profile_reset": [],
+            "tl_profile_reset": [], "tl_hbm_stream": [C.c_long, I, PD, PD],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -69,7 +69,10 @@ class TeaLeafSteps:
         if solver == "ppcg" and (self.inner_steps < 1 or self.presteps < 2):
             raise ValueError("ppcg needs inner_steps >= 1 and presteps >= 2")
         self.sweeps = 0                # PPCG inner sweeps of the last forward/backward
-        self.unconverged = 0           # solves that stopped at max_iters (as the reference, silently)
+        # solves that did not meet eps: at the iteration limit (max_iters; the
+        # reference carries on silently, tea_solve.f90:152) or, mixed
+        # precision, at the true residual's rounding floor
+        self.unconverged = self.maxiter_stops = self.floor_stops = 0
         self.mixed_inner_tol = mixed_inner_tol
         self.reflective = bool(reflective)
         self.iterations = 0            # solver iterations of the last forward/backward
@@ -103,7 +106,7 @@ class TeaLeafSteps:
                                inner_steps=self.inner_steps, presteps=self.presteps)
             self.iterations += st.inner_iters
             self.sweeps += st.sweeps
-            self.unconverged += st.reason != L.LOOP_CONVERGED
+            self._count(st.reason)
             return
         L.call("tl_set_ppcg_active", 0)
         L.call("tl_calc_residual")
@@ -117,9 +120,18 @@ class TeaLeafSteps:
                              epslim=0.0, presteps=0, rro=rro, old_error=rr)
             L.call("tl_cg_loop", C.byref(st), None, None)
             self.iterations += st.n_end
-            self.unconverged += st.reason != L.LOOP_CONVERGED
+            self._count(st.reason)
             return
         self._solve_ppcg(rro, rr, eps_abs)
+
+    def _count(self, reason):
+        if reason == L.LOOP_CONVERGED:
+            return
+        self.unconverged += 1
+        if reason == L.LOOP_FLOOR:
+            self.floor_stops += 1
+        else:
+            self.maxiter_stops += 1
 
     def _solve_ppcg(self, rro, rr, eps_abs):
         """FP64 PPCG as tea_solve.f90:160-330 runs it: CG until the switch at
@@ -137,7 +149,7 @@ class TeaLeafSteps:
         L.call("tl_cg_loop", C.byref(st), al.ctypes.data_as(pd), be.ctypes.data_as(pd))
         if st.reason != L.LOOP_SWITCH:
             self.iterations += st.n_end
-            self.unconverged += st.reason != L.LOOP_CONVERGED
+            self._count(st.reason)
             return
         n = st.n_end
         self.iterations += n - 1
@@ -159,7 +171,7 @@ class TeaLeafSteps:
         L.call("tl_ppcg_loop", C.byref(st2), ca.ctypes.data_as(pd), cb.ctypes.data_as(pd),
                self.inner_steps, theta, C.byref(sweeps))
         L.call("tl_set_ppcg_active", 0)
-        self.unconverged += st2.reason != L.LOOP_CONVERGED
+        self._count(st2.reason)
         self.iterations += st2.n_end - n + 1
         self.sweeps += sweeps.value
 
@@ -180,7 +192,7 @@ class TeaLeafSteps:
         L.call("tl_snapshot_load", self.E0, L.FIELD["energy1"])
         self.iterations = 0
         self.sweeps = 0
-        self.unconverged = 0
+        self.unconverged = self.maxiter_stops = self.floor_stops = 0
         for n in range(1, self.steps + 1):
             if (n - 1) % self.k == 0:
                 L.call("tl_snapshot_save", self.CK + (n - 1) // self.k, L.FIELD["energy1"])
@@ -203,7 +215,7 @@ class TeaLeafSteps:
         L.call("tl_snapshot_fill", self.GRAD, 0.0)
         self.iterations = 0
         self.sweeps = 0
-        self.unconverged = 0
+        self.unconverged = self.maxiter_stops = self.floor_stops = 0
         nseg = (self.steps + self.k - 1) // self.k
         for seg in range(nseg - 1, -1, -1):
             a, b = seg * self.k, min(seg * self.k + self.k, self.steps)

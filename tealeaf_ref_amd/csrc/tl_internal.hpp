@@ -69,7 +69,8 @@ struct State {
   double *hscal = nullptr;     // pinned host mirror of dscal
   void *ctl = nullptr;         // loop control block (device)
   void *hctl = nullptr;        // pinned host copies of the control block
-  double *hist = nullptr;      // loop history: rro, alpha, beta (3 x hist_len)
+  hipEvent_t loop_ev[2] = {nullptr, nullptr};  // batch markers of the loop driver
+  double *hist = nullptr;      // loop history: rro, alpha, beta, error (4 x hist_len)
   long hist_len = 0;
 
   // FP32 fields of the mixed-precision solver (allocated on first use):

@@ -3,6 +3,7 @@
 // tea_leaf_cheby.f90, tea_leaf_ppcg.f90, tea_leaf_jacobi.f90, update_halo.f90,
 // field_summary.f90).  Scalars come back synchronously, as the reference
 // wrappers return them.
+#include <cmath>
 #include <cstdlib>
 
 #include "tl_internal.hpp"
@@ -868,13 +869,19 @@ extern "C" int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int
   if (want_norm && !norm) return fail(TL_ERR_ARG, "tl_cheby_run: null norm");
   const int fused = cheby_run_fused(ch_alphas, ch_betas, step_first, count, want_norm, norm);
   if (fused < 0) return -fused;
-  if (fused) return TL_OK;
-  for (int i = 0; i < count; ++i) {
-    const int st = step_first + i - 1;
-    const int last = (i == count - 1) && want_norm;
-    int rc = cheby_step(1, 0.0, ch_alphas[st], ch_betas[st], last, norm);
-    if (rc) return rc;
+  if (!fused) {
+    for (int i = 0; i < count; ++i) {
+      const int st = step_first + i - 1;
+      const int last = (i == count - 1) && want_norm;
+      int rc = cheby_step(1, 0.0, ch_alphas[st], ch_betas[st], last, norm);
+      if (rc) return rc;
+    }
   }
+  // the residual check of the run (tea_solve.f90:273) on a NaN or Inf norm
+  if (want_norm && count > 0 && !std::isfinite(*norm))
+    return fail(TL_ERR_NONFINITE, "Chebyshev: non-finite residual norm at step " +
+                                      std::to_string(step_first + count - 1) +
+                                      " (NaN or Inf in the fields)");
   return TL_OK;
 }
 

@@ -24,6 +24,19 @@ struct Ctl {
   int stop, reason, n_end, pad;
   double error, old_error, rro;
 };
+// tl_init allocates 64 device bytes for the control block and 256 pinned
+// host bytes for its copies (two batches in flight and the final read)
+static_assert(sizeof(Ctl) <= 64 && 3 * sizeof(Ctl) <= 256, "control block size");
+
+// Device memory a fused loop keeps its scalars in: the control block, the
+// per-iteration histories indexed by iteration n (rro entering n, alpha and
+// beta of n, the exit-test error sqrt|rrn| of n) and the partial-sum slots.
+struct LoopMem {
+  Ctl *ctl;
+  double *hrro, *halpha, *hbeta, *herr;
+  double *parts;  // NPART slots of MAX_GRID doubles
+  double *part(int slot) const { return parts + (long)slot * MAX_GRID; }
+};
 
 struct LoopParams {
   double eps_abs;
@@ -34,42 +47,42 @@ struct LoopParams {
 
 bool use_nontemporal();
 int batch_size();
+// allocates the library's loop history for iterations up to n_last
 int ensure_loop_buffers(long n_last);
-double *h_rro();
-double *h_alpha();
-double *h_beta();
+// the library's own loop memory (after ensure_loop_buffers)
+LoopMem loop_mem();
 int copy_hist(double *dst, const double *src_dev, int lo, int hi);
+
+// TL_ERR_NONFINITE for a loop that stopped on a NaN or Inf reduction
+inline int nonfinite_fail(const char *solver, int n) {
+  return fail(TL_ERR_NONFINITE, std::string(solver) + ": non-finite reduction at iteration " +
+                                    std::to_string(n) + " (NaN or Inf in the fields)");
+}
 
 // Drive a loop: launch(n, first) queues iteration n; tail(n) queues the exit
 // test of iteration n-1.  Batches of iterations are queued two deep and the
-// control block is polled once per batch.
+// control block `dctl` is polled once per batch.  Allocates nothing.
 template <typename LAUNCH, typename TAIL>
-int drive(int n_first, int n_last, LAUNCH launch, TAIL tail, Ctl *out) {
+int drive(int n_first, int n_last, LAUNCH launch, TAIL tail, Ctl *out, const Ctl *dctl) {
   State &s = S();
   Ctl *hc = (Ctl *)s.hctl;
   const int B = batch_size();
-  hipEvent_t ev[2];
-  TL_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  TL_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  hipEvent_t *ev = s.loop_ev;
   int n = n_first, k = 0, rc = TL_OK;
   bool stopped = false;
   while (n <= n_last) {
     const int m = (n + B - 1 < n_last) ? n + B - 1 : n_last;
     for (int it = n; it <= m; ++it) {
       rc = launch(it, it == n_first);
-      if (rc) goto done;
+      if (rc) return rc;
     }
-    if (hipMemcpyAsync(&hc[k & 1], s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s.stream) !=
+    if (hipMemcpyAsync(&hc[k & 1], dctl, sizeof(Ctl), hipMemcpyDeviceToHost, s.stream) !=
             hipSuccess ||
-        hipEventRecord(ev[k & 1], s.stream) != hipSuccess) {
-      rc = fail(TL_ERR_HIP, "queueing the loop control block copy");
-      goto done;
-    }
+        hipEventRecord(ev[k & 1], s.stream) != hipSuccess)
+      return fail(TL_ERR_HIP, "queueing the loop control block copy");
     if (k >= 1) {
-      if (hipEventSynchronize(ev[(k - 1) & 1]) != hipSuccess) {
-        rc = fail(TL_ERR_HIP, "hipEventSynchronize");
-        goto done;
-      }
+      if (hipEventSynchronize(ev[(k - 1) & 1]) != hipSuccess)
+        return fail(TL_ERR_HIP, "hipEventSynchronize");
       if (hc[(k - 1) & 1].stop) {
         stopped = true;
         break;
@@ -80,18 +93,13 @@ int drive(int n_first, int n_last, LAUNCH launch, TAIL tail, Ctl *out) {
   }
   if (!stopped) {
     rc = tail(n_last + 1);
-    if (rc) goto done;
+    if (rc) return rc;
   }
-  if (hipMemcpyAsync(&hc[2], s.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-      hipStreamSynchronize(s.stream) != hipSuccess) {
-    rc = fail(TL_ERR_HIP, "reading the loop control block");
-    goto done;
-  }
+  if (hipMemcpyAsync(&hc[2], dctl, sizeof(Ctl), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+      hipStreamSynchronize(s.stream) != hipSuccess)
+    return fail(TL_ERR_HIP, "reading the loop control block");
   *out = hc[2];
-done:
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  return rc;
+  return TL_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -186,7 +194,8 @@ struct Head {
 __device__ inline Head loop_head(int n, int first, int tail, const LoopParams &lp,
                                  double rro_in, const double *__restrict__ prev_part, int nprev,
                                  Ctl *ctl, double *__restrict__ hrro,
-                                 double *__restrict__ hbeta, double *lds) {
+                                 double *__restrict__ hbeta, double *__restrict__ herr,
+                                 double *lds) {
   Head h;
   h.beta = 0.0;
   h.stop = false;
@@ -195,15 +204,20 @@ __device__ inline Head loop_head(int n, int first, int tail, const LoopParams &l
     const double rro = hrro[n - 1];
     h.beta = rrn / rro;
     const double err = sqrt(fabs(rrn));
-    const bool conv = fabs(err) < lp.eps_abs;
-    const bool sw = !conv && !tail && lp.check_switch && fabs(err) <= lp.epslim &&
+    // a NaN or Inf reduction (e.g. a zero density) stops the loop instead of
+    // iterating to the limit
+    const bool bad = !isfinite(rrn);
+    const bool conv = !bad && fabs(err) < lp.eps_abs;
+    const bool sw = !bad && !conv && !tail && lp.check_switch && fabs(err) <= lp.epslim &&
                     n >= lp.presteps;
-    h.stop = conv || sw || tail;
+    h.stop = bad || conv || sw || tail;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       hbeta[n - 1] = h.beta;
+      herr[n - 1] = err;
       if (h.stop) {
         ctl->stop = 1;
-        ctl->reason = conv ? TL_LOOP_CONVERGED : (sw ? TL_LOOP_SWITCH : TL_LOOP_MAXITERS);
+        ctl->reason = bad ? TL_LOOP_NONFINITE
+                          : (conv ? TL_LOOP_CONVERGED : (sw ? TL_LOOP_SWITCH : TL_LOOP_MAXITERS));
         ctl->n_end = sw ? n : n - 1;
         ctl->error = err;
         ctl->old_error = err;
@@ -217,6 +231,19 @@ __device__ inline Head loop_head(int n, int first, int tail, const LoopParams &l
     hrro[n] = rro_in;
   }
   return h;
+}
+
+// A non-finite p.w (or z.r) in the update pass of iteration n: block 0
+// records the stop; every block computed the same value and returns.
+__device__ inline bool nonfinite_stop(int n, double v, double v2, Ctl *ctl) {
+  if (isfinite(v) && isfinite(v2)) return false;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->stop = 1;
+    ctl->reason = TL_LOOP_NONFINITE;
+    ctl->n_end = n;
+    ctl->error = isfinite(v) ? v2 : v;
+  }
+  return true;
 }
 
 // First kernel of a CG or PPCG iteration n: p = src + beta p (of n-1; src is
@@ -235,11 +262,13 @@ __global__ __launch_bounds__(BLOCK, TL_W_WAVES) void k_loop_w(Geo G, int n, int 
                                                    const double *__restrict__ prev_part,
                                                    int nprev, double *__restrict__ out_part,
                                                    Ctl *ctl, double *__restrict__ hrro,
-                                                   double *__restrict__ hbeta, int dir) {
+                                                   double *__restrict__ hbeta,
+                                                   double *__restrict__ herr, int dir) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   if (!first && ctl->stop) return;
-  const Head h = loop_head(n, first, tail, lp, rro_in, prev_part, nprev, ctl, hrro, hbeta, lds);
+  const Head h =
+      loop_head(n, first, tail, lp, rro_in, prev_part, nprev, ctl, hrro, hbeta, herr, lds);
   const T beta = (T)h.beta;
   const bool upd = !first;
   const long P = G.pitch;
@@ -297,6 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
   const double pw = dev::sum_partials(prev_part, nprev, lds);
+  if (nonfinite_stop(n, pw, 0.0, ctl)) return;
   const double alpha = hrro[n] / pw;
   if (blockIdx.x == 0 && threadIdx.x == 0) halpha[n] = alpha;
   const T a = (T)alpha;
@@ -346,6 +376,7 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur_blk(
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
   const double pw = dev::sum_partials(prev_part, nprev, lds);
+  if (nonfinite_stop(n, pw, 0.0, ctl)) return;
   const double alpha = hrro[n] / pw;
   if (blockIdx.x == 0 && threadIdx.x == 0) halpha[n] = alpha;
   const T a = (T)alpha;
@@ -412,6 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double
   if (ctl->stop) return;
   const double pw = dev::sum_partials(part_pw, nprev, lds);
   const double rro = first ? rro_in : dev::sum_partials(part_zr, nprev, lds);
+  if (nonfinite_stop(n, pw, rro, ctl)) return;
   const double alpha = rro / pw;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     hrro[n] = rro;
@@ -520,17 +552,31 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
 }
 
 // Inner sweeps i and i+1 in one pass (tl_sweep2.hpp): reads sdin, rtin
-// (r on the first sweep), utemp; writes sdout, rtout, utemp.  first/last as
-// in k_pp_sweep, applying to sweep i / sweep i+1.
-template <bool NT, typename T, int V>
+// (r on the first sweep), utemp; writes sdout, rtout, utemp.  FIRST / LAST
+// as first_sweep / last_sweep in k_pp_sweep, applying to sweep i / sweep
+// i+1; DIAG: the Jacobi-diagonal preconditioner.  They are template
+// parameters, so that each of the eight kernels holds only the pointers and
+// registers it uses (the middle passes of an outer iteration need neither r,
+// r_store, z nor Mi).
+#ifndef TL_PF_PAIR
+#define TL_PF_PAIR 0
+#endif
+
+template <typename VT>
+struct SweepX {
+  VT ut, rv, rs;  // utemp, and r and r_store on the last sweep
+};
+
+template <bool NT, typename T, int V, int FIRST, int LAST, int DIAG>
 __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_WAVES64) void k_pp_sweep2(
-    Geo G, int first_sweep, int last_sweep, int want_rrn, T theta_r, T a1, T b1, T a2, T b2,
-    int diag, const T *__restrict__ src, const T *__restrict__ sdin, T *__restrict__ sdout,
-    const T *__restrict__ rtin, T *__restrict__ rtout, T *__restrict__ utemp,
-    const T *__restrict__ r, const T *__restrict__ r_store, T *__restrict__ z,
-    const T *__restrict__ Mi, const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry,
-    double *__restrict__ part_rrn, double *__restrict__ part_zr, const Ctl *ctl, int dir) {
+    Geo G, int want_rrn, T theta_r, T a1, T b1, T a2, T b2, const T *__restrict__ src,
+    const T *__restrict__ sdin, T *__restrict__ sdout, const T *__restrict__ rtin,
+    T *__restrict__ rtout, T *__restrict__ utemp, const T *__restrict__ r,
+    const T *__restrict__ r_store, T *__restrict__ z, const T *__restrict__ Mi,
+    const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry, double *__restrict__ part_rrn,
+    double *__restrict__ part_zr, const Ctl *ctl, int dir) {
   typedef typename dev::Vec<T, V>::type VT;
+  typedef SweepX<VT> X;
   constexpr bool ST = TL_ST_NT_SWEEP && sizeof(T) == 8;
   __shared__ double lds[BLOCK / WAVE];
   if (ctl && ctl->stop) return;
@@ -541,48 +587,55 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_W
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
     auto opA = [&](int col, int row) -> VT {
       const long i = (long)row * P + col;
-      return first_sweep ? dev::ldv<NT, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
+      return FIRST ? dev::ldv<NT, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
     };
     auto opA1 = [&](int col, int row) -> T {
       const long i = (long)row * P + col;
-      return first_sweep ? src[i] * theta_r : sdin[i];
+      return FIRST ? src[i] * theta_r : sdin[i];
     };
     auto rtA = [&](int col, int row) -> VT {
       const long i = (long)row * P + col;
-      return first_sweep ? dev::ldv<NT, T, V>(r + i) : dev::ldv<NT, T, V>(rtin + i);
+      return FIRST ? dev::ldv<NT, T, V>(r + i) : dev::ldv<NT, T, V>(rtin + i);
     };
     auto rtA1 = [&](int col, int row) -> T {
       const long i = (long)row * P + col;
-      return first_sweep ? r[i] : rtin[i];
+      return FIRST ? r[i] : rtin[i];
     };
-    dev::walk_pair<NT, T, V, diff_form<T>()>(
-        G, tt, Kx, Ky, Mi, diag, rx, ry, a1, b1, opA, opA1, rtA, rtA1,
-        [&](int y, const VT &s1, const VT &r1, const VT &s0, const VT &w, const VT &mi) {
+    auto xtra = [&](int col, int row) -> X {
+      const long i = (long)row * P + col;
+      X x;
+      x.ut = FIRST ? VT(0) : dev::ldv<NT, T, V>(utemp + i);
+      x.rv = LAST ? dev::ldv<NT, T, V>(r + i) : VT(0);
+      x.rs = (LAST && want_rrn) ? dev::ldv<NT, T, V>(r_store + i) : VT(0);
+      return x;
+    };
+    dev::walk_pair<NT, T, V, diff_form<T>(), DIAG, (TL_PF_PAIR != 0)>(
+        G, tt, Kx, Ky, Mi, rx, ry, a1, b1, opA, opA1, rtA, rtA1, xtra,
+        [&](int y, const VT &s1, const VT &r1, const VT &s0, const VT &w, const VT &mi,
+            const X &x) {
           if (tt.c >= G.nx) return;
           const long row = (long)y * P;
           const VT r2 = r1 - w;
-          const VT s2 = diag ? a2 * s1 + (b2 * mi) * r2 : a2 * s1 + b2 * r2;
-          const VT ut0 = first_sweep ? s0 : dev::ldv<NT, T, V>(utemp + row + tt.c);
+          const VT s2 = DIAG ? a2 * s1 + (b2 * mi) * r2 : a2 * s1 + b2 * r2;
+          const VT ut0 = FIRST ? s0 : x.ut;
           const VT ut = (ut0 + s1) + s2;
           dev::stv<(NT && ST), T, V>(rtout + row, tt.c, G.nx, r2);
           dev::stv<(NT && ST), T, V>(sdout + row, tt.c, G.nx, s2);
           dev::stv<(NT && (ST || (TL_ST_NT_UTEMP && sizeof(T) == 8))), T, V>(utemp + row, tt.c,
                                                                               G.nx, ut);
-          if (last_sweep) {
+          if (LAST) {
             dev::stv<(NT && ST), T, V>(z + row, tt.c, G.nx, ut);
-            const VT rv = dev::ldv<NT, T, V>(r + row + tt.c);
-            const VT rs = want_rrn ? dev::ldv<NT, T, V>(r_store + row + tt.c) : VT(0);
 #pragma unroll
             for (int e = 0; e < V; ++e)
               if (tt.c + e < G.nx) {
-                if (want_rrn) arrn += ((double)rv[e] - (double)rs[e]) * (double)ut[e];
-                azr += (double)ut[e] * (double)rv[e];
+                if (want_rrn) arrn += ((double)x.rv[e] - (double)x.rs[e]) * (double)ut[e];
+                azr += (double)ut[e] * (double)x.rv[e];
               }
           }
         },
         dir);
   }
-  if (last_sweep) {
+  if (LAST) {
     arrn = dev::block_sum(arrn, lds);
     azr = dev::block_sum(azr, lds);
     if (threadIdx.x == 0) {
@@ -590,6 +643,18 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_W
       part_zr[blockIdx.x] = azr;
     }
   }
+}
+
+// The two-sweep pass for a (first, last, diag) combination.
+template <bool NT, typename T, int V>
+auto sweep2_kernel(int first, int last, int diag) {
+  using K = decltype(&k_pp_sweep2<NT, T, V, 0, 0, 0>);
+  static const K tab[8] = {
+      &k_pp_sweep2<NT, T, V, 0, 0, 0>, &k_pp_sweep2<NT, T, V, 0, 0, 1>,
+      &k_pp_sweep2<NT, T, V, 0, 1, 0>, &k_pp_sweep2<NT, T, V, 0, 1, 1>,
+      &k_pp_sweep2<NT, T, V, 1, 0, 0>, &k_pp_sweep2<NT, T, V, 1, 0, 1>,
+      &k_pp_sweep2<NT, T, V, 1, 1, 0>, &k_pp_sweep2<NT, T, V, 1, 1, 1>};
+  return tab[(first ? 4 : 0) + (last ? 2 : 0) + (diag ? 1 : 0)];
 }
 
 // ---------------------------------------------------------------------------
@@ -626,16 +691,17 @@ inline bool sweep_pairs() {
 // between b.rtemp and b.w (w is dead between the update and the next w pass).
 // Passes walk in alternating directions starting with `dir`; without pairs,
 // both sweeps of what would be a pair take that pair's direction, so the two
-// modes produce the same bits.
+// modes produce the same bits.  The last sweep leaves the partials of
+// (r - r_store).z in slot P3 and of z.r in slot P2 of `parts`.
 template <typename T, int V>
 int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas, int inner,
-                 T theta_r, int want_rrn, const Ctl *ctl, int &sd, T *&rt_cur, int dir) {
+                 T theta_r, int want_rrn, const Ctl *ctl, const LoopMem &m, int &sd, T *&rt_cur,
+                 int dir) {
   State &s = S();
   const Geo G = b.G;
   const int nb = abs_blocks(G);
   const bool nt = use_nontemporal();
   auto k1 = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
-  auto k2 = nt ? k_pp_sweep2<true, T, V> : k_pp_sweep2<false, T, V>;
   const T *src = b.diag ? b.z : b.r;
   const bool pairs = sweep_pairs();
   for (int i = 1; i <= inner;) {
@@ -643,18 +709,19 @@ int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas
     prof_begin(2, &ev);
     if (pairs && i + 1 <= inner) {
       T *rt_next = rt_cur == b.rtemp ? b.w : b.rtemp;
-      hipLaunchKernelGGL(k2, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
-                         i + 1 == inner ? 1 : 0, want_rrn, theta_r, (T)alphas[i - 1],
-                         (T)betas[i - 1], (T)alphas[i], (T)betas[i], b.diag, src, b.sd[sd],
-                         b.sd[1 - sd], rt_cur, rt_next, b.utemp, b.r, b.r_store, b.z, b.Mi, b.Kx,
-                         b.Ky, b.rx, b.ry, part(P3), part(P2), ctl, dir);
+      auto k2 = nt ? sweep2_kernel<true, T, V>(i == 1, i + 1 == inner, b.diag)
+                   : sweep2_kernel<false, T, V>(i == 1, i + 1 == inner, b.diag);
+      hipLaunchKernelGGL(k2, dim3(nb), dim3(BLOCK), 0, s.stream, G, want_rrn, theta_r,
+                         (T)alphas[i - 1], (T)betas[i - 1], (T)alphas[i], (T)betas[i], src,
+                         b.sd[sd], b.sd[1 - sd], rt_cur, rt_next, b.utemp, b.r, b.r_store, b.z,
+                         b.Mi, b.Kx, b.Ky, b.rx, b.ry, m.part(P3), m.part(P2), ctl, dir);
       rt_cur = rt_next;
       i += 2;
     } else {
       hipLaunchKernelGGL(k1, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
                          i == inner ? 1 : 0, want_rrn, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
                          b.diag, src, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r, b.r_store,
-                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, part(P3), part(P2), ctl, dir);
+                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, m.part(P3), m.part(P2), ctl, dir);
       if (pairs || (i & 1) == 0) dir = -dir;
       i += 1;
       prof_end(2, ev);
@@ -670,13 +737,16 @@ int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas
 }
 
 // CG iterations (tea_solve.f90:341-369); see tl_cg_loop.  pin_first, when
-// given, replaces p[pcur] as the direction of the first iteration.
+// given, replaces p[pcur] as the direction of the first iteration.  The
+// scalars live in `mem` when given (which must hold histories for
+// iterations up to st->n_last), else in the library's loop memory.
 template <typename T, int V>
 int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, double *cg_betas,
-                const T *pin_first = nullptr) {
+                const T *pin_first = nullptr, const LoopMem *mem = nullptr) {
   State &s = S();
   int rc;
-  if ((rc = ensure_loop_buffers(st->n_last))) return rc;
+  if (!mem && (rc = ensure_loop_buffers(st->n_last))) return rc;
+  const LoopMem m = mem ? *mem : loop_mem();
   const Geo G = b.G;
   const int nb = abs_blocks(G);
   const bool nt = use_nontemporal();
@@ -689,7 +759,6 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
   lp.epslim = st->epslim;
   lp.check_switch = st->check_switch;
   lp.presteps = st->presteps;
-  Ctl *ctl = (Ctl *)s.ctl;
   auto launch = [&](int n, bool first) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
     const T *pin = (first && pin_first) ? pin_first : b.p[cin];
@@ -697,17 +766,17 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
     prof_begin(0, &ev);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
                        st->rro, b.cg_src, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
-                       part(P2), nb, part(P1), ctl, h_rro(), h_beta(), 1);
+                       m.part(P2), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, 1);
     prof_end(0, ev);
     prof_begin(1, &ev);
     if (b.block)
       hipLaunchKernelGGL(kub, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
-                         b.w, b.z, b.Di, b.Ky, b.cp, b.bfp, b.ry, part(P1), nb, part(P2), ctl,
-                         h_rro(), h_alpha(), 1);
+                         b.w, b.z, b.Di, b.Ky, b.cp, b.bfp, b.ry, m.part(P1), nb, m.part(P2),
+                         m.ctl, m.hrro, m.halpha, 1);
     else
       hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
-                         b.w, b.z, b.Mi, b.diag, part(P1), nb, part(P2), ctl, h_rro(), h_alpha(),
-                         1);
+                         b.w, b.z, b.Mi, b.diag, m.part(P1), nb, m.part(P2), m.ctl, m.hrro,
+                         m.halpha, 1);
     prof_end(1, ev);
     TL_HIP(hipGetLastError());
     return TL_OK;
@@ -715,24 +784,25 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
   auto tail = [&](int n) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, st->rro, b.cg_src,
-                       b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P2), nb,
-                       part(P1), ctl, h_rro(), h_beta(), 1);
+                       b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, m.part(P2), nb,
+                       m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, 1);
     TL_HIP(hipGetLastError());
     return TL_OK;
   };
   Ctl res;
-  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res))) return rc;
+  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res, m.ctl))) return rc;
   st->n_end = res.n_end;
   st->reason = res.reason;
   st->error = res.error;
   st->rro = res.rro;
   st->old_error = res.old_error;
+  if (res.reason == TL_LOOP_NONFINITE) return nonfinite_fail("CG", res.n_end);
   // the launch that stopped the loop applied the last p update
   const int n_det = res.reason == TL_LOOP_SWITCH ? res.n_end : res.n_end + 1;
   *b.pcur = c0 ^ ((n_det - st->n_first + 1) & 1);
   const int last_done = res.reason == TL_LOOP_SWITCH ? res.n_end - 1 : res.n_end;
-  if ((rc = copy_hist(cg_alphas, h_alpha(), st->n_first, last_done))) return rc;
-  if ((rc = copy_hist(cg_betas, h_beta(), st->n_first, last_done))) return rc;
+  if ((rc = copy_hist(cg_alphas, m.halpha, st->n_first, last_done))) return rc;
+  if ((rc = copy_hist(cg_betas, m.hbeta, st->n_first, last_done))) return rc;
   return TL_OK;
 }
 
@@ -746,6 +816,7 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   State &s = S();
   int rc;
   if ((rc = ensure_loop_buffers(st->n_last))) return rc;
+  const LoopMem m = loop_mem();
   const Geo G = b.G;
   const int nb = abs_blocks(G);
   const bool nt = use_nontemporal();
@@ -753,7 +824,6 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
   auto kur = nt ? k_pp_ur<true, T, V> : k_pp_ur<false, T, V>;
   const int c0 = *b.pcur;
   int sd = *b.sdcur;
-  Ctl *ctl = (Ctl *)s.ctl;
   LoopParams lp;
   lp.eps_abs = st->eps_abs;
   lp.epslim = 0.0;
@@ -771,32 +841,33 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
     hipEvent_t ev;
     prof_begin(5, &ev);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
-                       rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3),
-                       nb, part(P1), ctl, h_rro(), h_beta(), d);
+                       rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
+                       m.part(P3), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, d);
     prof_end(5, ev);
     prof_begin(6, &ev);
     hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, rro_first,
-                       b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag, part(P1),
-                       part(P2), nb, ctl, h_rro(), h_alpha(), d > 0 ? 1 : 0);
+                       b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag, m.part(P1),
+                       m.part(P2), nb, m.ctl, m.hrro, m.halpha, d > 0 ? 1 : 0);
     prof_end(6, ev);
     T *rt_cur = b.rtemp;
-    return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, ctl, sd, rt_cur, d);
+    return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, m.ctl, m, sd, rt_cur, d);
   };
   auto tail = [&](int n) -> int {
     const int cin = c0 ^ ((n - st->n_first) & 1);
     hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, 1, lp, rro_first, b.z,
-                       b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, part(P3), nb,
-                       part(P1), ctl, h_rro(), h_beta(), 1);
+                       b.p[cin], b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry, m.part(P3), nb,
+                       m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, 1);
     TL_HIP(hipGetLastError());
     return TL_OK;
   };
   Ctl res;
-  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res))) return rc;
+  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res, m.ctl))) return rc;
   st->n_end = res.n_end;
   st->reason = res.reason;
   st->error = res.error;
   st->rro = res.rro;
   st->old_error = res.old_error;
+  if (res.reason == TL_LOOP_NONFINITE) return nonfinite_fail("PPCG", res.n_end);
   const int executed = res.n_end - st->n_first + 1;  // outer iterations that ran
   if (inner_iters) *inner_iters += executed * inner;
   *b.pcur = c0 ^ ((executed + 1) & 1);               // + the stopping launch's p update
@@ -813,8 +884,9 @@ int run_ppcg_precondition(const LoopBufs<T> &b, const double *alphas, const doub
                           int inner, double theta) {
   int sd = *b.sdcur;
   T *rt_cur = b.rtemp;
+  const LoopMem m = loop_mem();
   const int rc = queue_sweeps<T, V>(b, alphas, betas, inner, (T)(1.0 / theta), 0,
-                                    (const Ctl *)nullptr, sd, rt_cur, 1);
+                                    (const Ctl *)nullptr, m, sd, rt_cur, 1);
   if (rc) return rc;
   *b.sdcur = sd;
   return TL_OK;

@@ -33,6 +33,30 @@ namespace {
 // Jacobi iteration n (tea_solve.f90:375-379): exit test of n-1, then the
 // sweep from u[in] to u[out] with its |u - un| partials (parity-buffered).
 // ---------------------------------------------------------------------------
+// Exit test of iteration n-1 at the head of iteration n (none on the first):
+// true when the loop stops (converged, non-finite error, or stopped before).
+// herr[n-1] gets the error sqrt|sum|u - un||.
+__device__ inline bool jacobi_head(int n, int first, double eps_abs,
+                                   const double *__restrict__ prev_part, int nprev, Ctl *ctl,
+                                   double *__restrict__ herr, double *lds) {
+  if (first) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->stop = 0;
+    return false;
+  }
+  if (ctl->stop) return true;
+  const double e = sqrt(fabs(dev::sum_partials(prev_part, nprev, lds)));
+  const bool conv = fabs(e) < eps_abs;
+  if (blockIdx.x == 0 && threadIdx.x == 0) herr[n - 1] = e;
+  if (!conv && isfinite(e)) return false;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->stop = 1;
+    ctl->reason = conv ? TL_LOOP_CONVERGED : TL_LOOP_NONFINITE;
+    ctl->n_end = n - 1;
+    ctl->error = e;
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_jl(Geo G, int n, int first, double eps_abs,
                                                const double *__restrict__ un,
                                                double *__restrict__ uout,
@@ -40,21 +64,10 @@ __global__ __launch_bounds__(BLOCK) void k_jl(Geo G, int n, int first, double ep
                                                const double *__restrict__ Kx,
                                                const double *__restrict__ Ky, double rx, double ry,
                                                const double *__restrict__ prev_part, int nprev,
-                                               double *__restrict__ out_part, Ctl *ctl) {
+                                               double *__restrict__ out_part, Ctl *ctl,
+                                               double *__restrict__ herr) {
   __shared__ double lds[BLOCK / WAVE];
-  if (!first) {
-    if (ctl->stop) return;
-    const double e = sqrt(fabs(dev::sum_partials(prev_part, nprev, lds)));
-    const bool conv = fabs(e) < eps_abs;
-    if (conv) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->stop = 1; ctl->reason = TL_LOOP_CONVERGED; ctl->n_end = n - 1; ctl->error = e;
-      }
-      return;
-    }
-  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctl->stop = 0;
-  }
+  if (jacobi_head(n, first, eps_abs, prev_part, nprev, ctl, herr, lds)) return;
   double acc = 0.0;
   TILE_LOOP(G)
   dev::walk<ZERO>(G, x, tt.y0, tt.y1, [&](int xx, int yy) { return un[IDX(yy, xx)]; }, Kx, Ky,
@@ -85,22 +98,10 @@ __global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double 
                                                 const double *__restrict__ Ky, double rx,
                                                 double ry, const double *__restrict__ prev_part,
                                                 int nprev, double *__restrict__ out_part, Ctl *ctl,
-                                                int dir) {
+                                                double *__restrict__ herr, int dir) {
   typedef typename dev::Vec<double, 2>::type VT;
   __shared__ double lds[BLOCK / WAVE];
-  if (!first) {
-    if (ctl->stop) return;
-    const double e = sqrt(fabs(dev::sum_partials(prev_part, nprev, lds)));
-    const bool conv = fabs(e) < eps_abs;
-    if (conv) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->stop = 1; ctl->reason = TL_LOOP_CONVERGED; ctl->n_end = n - 1; ctl->error = e;
-      }
-      return;
-    }
-  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctl->stop = 0;
-  }
+  if (jacobi_head(n, first, eps_abs, prev_part, nprev, ctl, herr, lds)) return;
   const long P = G.pitch;
   double acc = 0.0;
   dev::TileIter it(G.tiles, G.blocks);
@@ -138,22 +139,26 @@ __global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double 
 // at_limit, records the stop at n-1 in any case.
 __device__ inline bool jacobi_pair_exit(int n, double eps_abs, const double *__restrict__ pa,
                                         const double *__restrict__ pb, int nprev, Ctl *ctl,
-                                        double *lds, bool at_limit) {
+                                        double *__restrict__ herr, double *lds, bool at_limit) {
   if (pa) {
     const double ea = sqrt(fabs(dev::sum_partials(pa, nprev, lds)));
-    if (fabs(ea) < eps_abs) {
+    const bool conv_a = fabs(ea) < eps_abs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) herr[n - 2] = ea;
+    if (conv_a || !isfinite(ea)) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->stop = 1; ctl->reason = TL_LOOP_CONVERGED; ctl->n_end = n - 2; ctl->error = ea;
+        ctl->stop = 1; ctl->reason = conv_a ? TL_LOOP_CONVERGED : TL_LOOP_NONFINITE;
+        ctl->n_end = n - 2; ctl->error = ea;
       }
       return true;
     }
   }
   const double eb = sqrt(fabs(dev::sum_partials(pb, nprev, lds)));
-  const bool conv = fabs(eb) < eps_abs;
-  if (conv || at_limit) {
+  const bool conv = fabs(eb) < eps_abs, bad = !isfinite(eb);
+  if (blockIdx.x == 0 && threadIdx.x == 0) herr[n - 1] = eb;
+  if (conv || bad || at_limit) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       ctl->stop = 1;
-      ctl->reason = conv ? TL_LOOP_CONVERGED : TL_LOOP_MAXITERS;
+      ctl->reason = conv ? TL_LOOP_CONVERGED : (bad ? TL_LOOP_NONFINITE : TL_LOOP_MAXITERS);
       ctl->n_end = n - 1;
       ctl->error = eb;
     }
@@ -178,12 +183,13 @@ __global__ __launch_bounds__(BLOCK, TL_JACOBI_PAIR_WAVES) void k_jl_pair(
     double *__restrict__ umid, double *__restrict__ uout, const double *__restrict__ u0,
     const double *__restrict__ Kx, const double *__restrict__ Ky, double rx, double ry,
     const double *__restrict__ prev_a, const double *__restrict__ prev_b, int nprev,
-    double *__restrict__ part_a, double *__restrict__ part_b, Ctl *ctl, int dir) {
+    double *__restrict__ part_a, double *__restrict__ part_b, Ctl *ctl,
+    double *__restrict__ herr, int dir) {
   typedef typename dev::Vec<double, 2>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   if (!first) {
     if (ctl->stop) return;
-    if (jacobi_pair_exit(n, eps_abs, prev_a, prev_b, nprev, ctl, lds, false)) return;
+    if (jacobi_pair_exit(n, eps_abs, prev_a, prev_b, nprev, ctl, herr, lds, false)) return;
   } else if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->stop = 0;
   }
@@ -228,20 +234,22 @@ __global__ __launch_bounds__(BLOCK, TL_JACOBI_PAIR_WAVES) void k_jl_pair(
 // iteration).
 __global__ void k_jl_pair_test(int n, double eps_abs, const double *__restrict__ pa,
                                const double *__restrict__ pb, int nprev, Ctl *ctl,
-                               int at_limit) {
+                               double *__restrict__ herr, int at_limit) {
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
-  jacobi_pair_exit(n, eps_abs, pa, pb, nprev, ctl, lds, at_limit != 0);
+  jacobi_pair_exit(n, eps_abs, pa, pb, nprev, ctl, herr, lds, at_limit != 0);
 }
 
 __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev_part, int nprev,
-                          Ctl *ctl) {
+                          Ctl *ctl, double *__restrict__ herr) {
   __shared__ double lds[BLOCK / WAVE];
   if (ctl->stop) return;
   const double e = sqrt(fabs(dev::sum_partials(prev_part, nprev, lds)));
   if (threadIdx.x == 0) {
+    herr[n - 1] = e;
     ctl->stop = 1;
-    ctl->reason = fabs(e) < eps_abs ? TL_LOOP_CONVERGED : TL_LOOP_MAXITERS;
+    ctl->reason = fabs(e) < eps_abs ? TL_LOOP_CONVERGED
+                                    : (isfinite(e) ? TL_LOOP_MAXITERS : TL_LOOP_NONFINITE);
     ctl->n_end = n - 1;
     ctl->error = e;
   }
@@ -514,10 +522,110 @@ LoopBufs<double> fp64_bufs() {
   return b;
 }
 
+// A non-finite reduction on the per-routine path of a loop entry point.
+int per_routine_nonfinite(tl_loop_state *st, const char *solver, int n, double v) {
+  st->n_end = n;
+  st->reason = TL_LOOP_NONFINITE;
+  st->error = v;
+  return nonfinite_fail(solver, n);
+}
+
 int check_state(tl_loop_state *st) {
   if (!st) return fail(TL_ERR_ARG, "null loop state");
   if (st->n_first < 1 || st->n_last < st->n_first)
     return fail(TL_ERR_ARG, "loop: need 1 <= n_first <= n_last");
+  return TL_OK;
+}
+
+// The per-routine loop paths keep their exit-test errors on the host; they
+// go to the device history, where tl_loop_errors finds the fused loops'.
+template <typename F>
+int with_errors(tl_loop_state *st, F run) {
+  int rc = ensure_loop_buffers(st->n_last);
+  if (rc) return rc;
+  std::vector<double> errs;
+  rc = run(errs);
+  if (!errs.empty()) {
+    State &s = S();
+    TL_HIP(hipMemcpyAsync(loop_mem().herr + st->n_first, errs.data(),
+                          sizeof(double) * errs.size(), hipMemcpyHostToDevice, s.stream));
+    TL_HIP(hipStreamSynchronize(s.stream));
+  }
+  return rc;
+}
+
+// CG on a chunk with internal faces: the per-routine path, one iteration at
+// a time; errs[n - n_first] gets the exit-test error of iteration n.
+int cg_loop_per_routine(tl_loop_state *st, double *cg_alphas, double *cg_betas,
+                        std::vector<double> &errs) {
+  int rc;
+  double rro = st->rro, old_error = st->old_error, err = 0.0;
+  for (int n = st->n_first; n <= st->n_last; ++n) {
+    if (n > st->n_first && st->check_switch && fabs(old_error) <= st->epslim &&
+        n >= st->presteps) {
+      st->n_end = n; st->reason = TL_LOOP_SWITCH; st->rro = rro; st->old_error = old_error;
+      st->error = err;
+      return TL_OK;
+    }
+    double pw, rrn;
+    if ((rc = tl_cg_calc_w(&pw))) return rc;
+    if (!std::isfinite(pw)) return per_routine_nonfinite(st, "CG", n, pw);
+    const double alpha = rro / pw;
+    if (cg_alphas) cg_alphas[n] = alpha;
+    if ((rc = tl_cg_calc_ur(alpha, &rrn))) return rc;
+    if (!std::isfinite(rrn)) return per_routine_nonfinite(st, "CG", n, rrn);
+    const double beta = rrn / rro;
+    if (cg_betas) cg_betas[n] = beta;
+    if ((rc = tl_cg_calc_p(beta))) return rc;
+    rro = rrn;
+    err = sqrt(fabs(rrn));
+    errs.push_back(err);
+    if (fabs(err) < st->eps_abs) {
+      st->n_end = n; st->reason = TL_LOOP_CONVERGED; st->rro = rro; st->error = err;
+      return TL_OK;
+    }
+    old_error = err;
+  }
+  st->n_end = st->n_last; st->reason = TL_LOOP_MAXITERS; st->rro = rro; st->error = err;
+  st->old_error = old_error;
+  return TL_OK;
+}
+
+// PPCG with the block preconditioner (or no inner steps): the per-routine
+// path (tea_solve.f90:294-330), one outer iteration at a time.
+int ppcg_loop_per_routine(tl_loop_state *st, const double *alphas, const double *betas, int inner,
+                          double theta, int *inner_iters, std::vector<double> &errs) {
+  int rc;
+  double rro = st->rro, err = 0.0;
+  for (int n = st->n_first; n <= st->n_last; ++n) {
+    double pw, rrn, dummy;
+    if ((rc = tl_cg_calc_w(&pw))) return rc;
+    if ((rc = tl_ppcg_store_r())) return rc;
+    if ((rc = tl_ppcg_calc_zrnorm(&rro))) return rc;
+    if (!std::isfinite(pw) || !std::isfinite(rro))
+      return per_routine_nonfinite(st, "PPCG", n, std::isfinite(pw) ? rro : pw);
+    const double alpha = rro / pw;
+    if ((rc = tl_cg_calc_ur(alpha, &dummy))) return rc;
+    if (inner >= 0) {
+      if ((rc = tl_ppcg_init_sd(theta))) return rc;
+      if ((rc = tl_ppcg_inner(alphas, betas, 1, inner))) return rc;
+      if ((rc = tl_ppcg_update_z())) return rc;
+    }
+    if (inner_iters) *inner_iters += inner;
+    if ((rc = tl_ppcg_calc_rrn(&rrn))) return rc;
+    if (!std::isfinite(rrn)) return per_routine_nonfinite(st, "PPCG", n, rrn);
+    const double beta = rrn / rro;
+    if ((rc = tl_cg_calc_p(beta))) return rc;
+    rro = rrn;
+    err = sqrt(fabs(rrn));
+    errs.push_back(err);
+    if (fabs(err) < st->eps_abs) {
+      st->n_end = n; st->reason = TL_LOOP_CONVERGED; st->rro = rro; st->error = err;
+      return TL_OK;
+    }
+    st->old_error = err;
+  }
+  st->n_end = st->n_last; st->reason = TL_LOOP_MAXITERS; st->rro = rro; st->error = err;
   return TL_OK;
 }
 
@@ -554,23 +662,31 @@ int batch_size() {
 
 int ensure_loop_buffers(long n_last) {
   State &s = S();
-  if (!s.ctl) {
-    TL_HIP(hipMalloc(&s.ctl, sizeof(Ctl)));
-    TL_HIP(hipMemsetAsync(s.ctl, 0, sizeof(Ctl), s.stream));
-    TL_HIP(hipHostMalloc(&s.hctl, 4 * sizeof(Ctl), hipHostMallocDefault));
-  }
   const long need = n_last + 4;
   if (s.hist_len < need) {
-    if (s.hist) TL_HIP(hipFree(s.hist));
-    TL_HIP(hipMalloc((void **)&s.hist, sizeof(double) * 3 * need));
+    // forget the old buffer before allocating, so that a failed allocation
+    // leaves no stale pointer or length behind
+    double *old = s.hist;
+    s.hist = nullptr;
+    s.hist_len = 0;
+    if (old) TL_HIP(hipFree(old));
+    TL_HIP(hipMalloc((void **)&s.hist, sizeof(double) * 4 * need));
     s.hist_len = need;
   }
   return TL_OK;
 }
 
-double *h_rro() { return S().hist; }
-double *h_alpha() { return S().hist + S().hist_len; }
-double *h_beta() { return S().hist + 2 * S().hist_len; }
+LoopMem loop_mem() {
+  State &s = S();
+  LoopMem m;
+  m.ctl = (Ctl *)s.ctl;
+  m.hrro = s.hist;
+  m.halpha = s.hist ? s.hist + s.hist_len : nullptr;
+  m.hbeta = s.hist ? s.hist + 2 * s.hist_len : nullptr;
+  m.herr = s.hist ? s.hist + 3 * s.hist_len : nullptr;
+  m.parts = s.partials;
+  return m;
+}
 
 int copy_hist(double *dst, const double *src_dev, int lo, int hi) {
   if (!dst || hi < lo) return TL_OK;
@@ -591,37 +707,9 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
   int rc = check_state(st);
   if (rc) return rc;
   State &s = S();
-  if (!s.zb_all) {
-    // a chunk with internal faces: the per-routine path, one iteration at a
-    // time
-    double rro = st->rro, old_error = st->old_error, err = 0.0;
-    for (int n = st->n_first; n <= st->n_last; ++n) {
-      if (n > st->n_first && st->check_switch && fabs(old_error) <= st->epslim &&
-          n >= st->presteps) {
-        st->n_end = n; st->reason = TL_LOOP_SWITCH; st->rro = rro; st->old_error = old_error;
-        st->error = err;
-        return TL_OK;
-      }
-      double pw, rrn;
-      if ((rc = tl_cg_calc_w(&pw))) return rc;
-      const double alpha = rro / pw;
-      if (cg_alphas) cg_alphas[n] = alpha;
-      if ((rc = tl_cg_calc_ur(alpha, &rrn))) return rc;
-      const double beta = rrn / rro;
-      if (cg_betas) cg_betas[n] = beta;
-      if ((rc = tl_cg_calc_p(beta))) return rc;
-      rro = rrn;
-      err = sqrt(fabs(rrn));
-      if (fabs(err) < st->eps_abs) {
-        st->n_end = n; st->reason = TL_LOOP_CONVERGED; st->rro = rro; st->error = err;
-        return TL_OK;
-      }
-      old_error = err;
-    }
-    st->n_end = st->n_last; st->reason = TL_LOOP_MAXITERS; st->rro = rro; st->error = err;
-    st->old_error = old_error;
-    return TL_OK;
-  }
+  if (!s.zb_all) return with_errors(st, [&](std::vector<double> &errs) {
+    return cg_loop_per_routine(st, cg_alphas, cg_betas, errs);
+  });
   LoopBufs<double> bb = fp64_bufs();
   return run_cg_loop<double, TL_V64>(bb, st, cg_alphas, cg_betas);
 }
@@ -633,36 +721,10 @@ extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const doubl
   if (rc) return rc;
   if (!alphas || !betas) return fail(TL_ERR_ARG, "tl_ppcg_loop: null coefficients");
   State &s = S();
-  if (s.precond == TL_PREC_JAC_BLOCK || inner < 1) {
-    // per-routine path (tea_solve.f90:294-330), one outer iteration at a time
-    double rro = st->rro, err = 0.0;
-    for (int n = st->n_first; n <= st->n_last; ++n) {
-      double pw, rrn, dummy;
-      if ((rc = tl_cg_calc_w(&pw))) return rc;
-      if ((rc = tl_ppcg_store_r())) return rc;
-      if ((rc = tl_ppcg_calc_zrnorm(&rro))) return rc;
-      const double alpha = rro / pw;
-      if ((rc = tl_cg_calc_ur(alpha, &dummy))) return rc;
-      if (inner >= 0) {
-        if ((rc = tl_ppcg_init_sd(theta))) return rc;
-        if ((rc = tl_ppcg_inner(alphas, betas, 1, inner))) return rc;
-        if ((rc = tl_ppcg_update_z())) return rc;
-      }
-      if (inner_iters) *inner_iters += inner;
-      if ((rc = tl_ppcg_calc_rrn(&rrn))) return rc;
-      const double beta = rrn / rro;
-      if ((rc = tl_cg_calc_p(beta))) return rc;
-      rro = rrn;
-      err = sqrt(fabs(rrn));
-      if (fabs(err) < st->eps_abs) {
-        st->n_end = n; st->reason = TL_LOOP_CONVERGED; st->rro = rro; st->error = err;
-        return TL_OK;
-      }
-      st->old_error = err;
-    }
-    st->n_end = st->n_last; st->reason = TL_LOOP_MAXITERS; st->rro = rro; st->error = err;
-    return TL_OK;
-  }
+  if (s.precond == TL_PREC_JAC_BLOCK || inner < 1)
+    return with_errors(st, [&](std::vector<double> &errs) {
+      return ppcg_loop_per_routine(st, alphas, betas, inner, theta, inner_iters, errs);
+    });
   // z.r entering the first iteration (tea_leaf_ppcg_calc_zrnorm); later
   // iterations take it from the last inner sweep of the one before
   double rro_first;
@@ -687,6 +749,7 @@ int jacobi_loop_pairs(tl_loop_state *st) {
   const int c0 = s.ucur;
   double *mid = s.p[1 - s.pcur].p11;  // not visible through TL_FIELD_P
   Ctl *ctl = (Ctl *)s.ctl;
+  double *herr = loop_mem().herr;
   const bool nt = use_nontemporal();
   auto kp = nt ? k_jl_pair<true> : k_jl_pair<false>;
   auto kv = nt ? k_jl_v<true> : k_jl_v<false>;
@@ -705,18 +768,18 @@ int jacobi_loop_pairs(tl_loop_state *st) {
     if (!is_single(j)) {
       hipLaunchKernelGGL(kp, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
                          st->eps_abs, s.u[in].p11, mid, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
-                         s.Ky.p11, s.rx, s.ry, pa, pb, nb, slot_a(j), slot_b(j), ctl, dir);
+                         s.Ky.p11, s.rx, s.ry, pa, pb, nb, slot_a(j), slot_b(j), ctl, herr, dir);
     } else if (first) {
       hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 1, st->eps_abs,
                          s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx,
-                         s.ry, slot_b(j), nb, slot_b(j), ctl, dir);
+                         s.ry, slot_b(j), nb, slot_b(j), ctl, herr, dir);
     } else {
       // exit test of the previous pair, then one iteration
       hipLaunchKernelGGL(k_jl_pair_test, dim3(1), dim3(BLOCK), 0, s.stream, n, st->eps_abs, pa,
-                         pb, nb, ctl, 0);
+                         pb, nb, ctl, herr, 0);
       hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, st->eps_abs,
                          s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx,
-                         s.ry, pb, nb, slot_b(j), ctl, dir);
+                         s.ry, pb, nb, slot_b(j), ctl, herr, dir);
     }
     prof_end(4, ev);
     TL_HIP(hipGetLastError());
@@ -725,16 +788,17 @@ int jacobi_loop_pairs(tl_loop_state *st) {
   auto tail = [&](int n) -> int {
     const int j = (n_last - n_first) / 2;  // the last launch
     hipLaunchKernelGGL(k_jl_pair_test, dim3(1), dim3(BLOCK), 0, s.stream, n, st->eps_abs,
-                       is_single(j) ? nullptr : slot_a(j), slot_b(j), nb, ctl, 1);
+                       is_single(j) ? nullptr : slot_a(j), slot_b(j), nb, ctl, herr, 1);
     TL_HIP(hipGetLastError());
     return TL_OK;
   };
   Ctl res;
   int rc;
-  if ((rc = drive(n_first, n_last, launch, tail, &res))) return rc;
+  if ((rc = drive(n_first, n_last, launch, tail, &res, ctl))) return rc;
   st->n_end = res.n_end;
   st->reason = res.reason;
   st->error = res.error;
+  if (res.reason == TL_LOOP_NONFINITE) return nonfinite_fail("Jacobi", res.n_end);
   const int off = res.n_end - n_first, j = off / 2, in = c0 ^ (j & 1);
   s.ucur = 1 - in;  // launch j wrote its last iteration to u[1 - in_j]
   if ((off & 1) == 0 && !is_single(j)) {
@@ -746,6 +810,15 @@ int jacobi_loop_pairs(tl_loop_state *st) {
   return TL_OK;
 }
 }  // namespace
+
+extern "C" int tl_loop_errors(int lo, int hi, double *err) {
+  TL_REQUIRE_INIT();
+  State &s = S();
+  if (!err) return fail(TL_ERR_ARG, "tl_loop_errors: null array");
+  if (lo < 1 || hi < lo || !s.hist || hi + 1 >= s.hist_len)
+    return fail(TL_ERR_ARG, "tl_loop_errors: iterations outside the last loop's history");
+  return copy_hist(err, loop_mem().herr, lo, hi);
+}
 
 extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   TL_REQUIRE_INIT();
@@ -762,6 +835,7 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int u0c = s.ucur;
   Ctl *ctl = (Ctl *)s.ctl;
+  double *herr = loop_mem().herr;
   auto kv = use_nontemporal() ? k_jl_v<true> : k_jl_v<false>;
   auto launch = [&](int n, bool first) -> int {
     const int in = u0c ^ ((n - st->n_first) & 1);
@@ -773,12 +847,12 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
       hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
                          st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
                          s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb, part(n & 1 ? P4 : P5), ctl,
-                         dir);
+                         herr, dir);
     } else {
       hipLaunchKernelGGL(k_jl, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
                          st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
                          s.Ky.p11, s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb,
-                         part(n & 1 ? P4 : P5), ctl);
+                         part(n & 1 ? P4 : P5), ctl, herr);
     }
     prof_end(4, ev);
     TL_HIP(hipGetLastError());
@@ -786,15 +860,16 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   };
   auto tail = [&](int n) -> int {
     hipLaunchKernelGGL(k_jl_tail, dim3(1), dim3(BLOCK), 0, s.stream, n, st->eps_abs,
-                       part((n - 1) & 1 ? P4 : P5), nb, ctl);
+                       part((n - 1) & 1 ? P4 : P5), nb, ctl, herr);
     TL_HIP(hipGetLastError());
     return TL_OK;
   };
   Ctl res;
-  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res))) return rc;
+  if ((rc = drive(st->n_first, st->n_last, launch, tail, &res, ctl))) return rc;
   st->n_end = res.n_end;
   st->reason = res.reason;
   st->error = res.error;
+  if (res.reason == TL_LOOP_NONFINITE) return nonfinite_fail("Jacobi", res.n_end);
   s.ucur = u0c ^ ((res.n_end - st->n_first + 1) & 1);
   return TL_OK;
 }

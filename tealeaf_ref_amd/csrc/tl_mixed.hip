@@ -425,6 +425,6 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
     stalls = res > 0.5 * prev ? stalls + 1 : 0;
     st->final_residual = res;
   }
-  st->reason = res <= target ? TL_LOOP_CONVERGED : TL_LOOP_MAXITERS;
+  st->reason = res <= target ? TL_LOOP_CONVERGED : (stalls >= 2 ? TL_LOOP_FLOOR : TL_LOOP_MAXITERS);
   return TL_OK;
 }

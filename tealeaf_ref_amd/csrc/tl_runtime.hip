@@ -187,6 +187,8 @@ extern "C" int tl_finalize(void) {
   if (s.ctl) (void)hipFree(s.ctl);
   if (s.hctl) (void)hipHostFree(s.hctl);
   if (s.hist) (void)hipFree(s.hist);
+  for (hipEvent_t e : s.loop_ev)
+    if (e) (void)hipEventDestroy(e);
   if (s.f32) (void)hipFree(s.f32);
   for (double *q : s.snaps) (void)hipFree(q);
   for (auto &ev : s.ev_pending) {
@@ -230,6 +232,13 @@ extern "C" int tl_init(int nx, int ny, int halo, int device) {
   TL_HIP(hipMemsetAsync(s.partials, 0, sizeof(double) * MAX_GRID * NPART, s.stream));
   TL_HIP(hipMalloc((void **)&s.dscal, sizeof(double) * 64));
   TL_HIP(hipHostMalloc((void **)&s.hscal, sizeof(double) * 64, hipHostMallocDefault));
+  // the fused loops' control block, its host copies and the driver's events
+  // (set up here so that a loop call allocates nothing but its history)
+  TL_HIP(hipMalloc(&s.ctl, 64));
+  TL_HIP(hipMemsetAsync(s.ctl, 0, 64, s.stream));
+  TL_HIP(hipHostMalloc(&s.hctl, 256, hipHostMallocDefault));
+  TL_HIP(hipEventCreateWithFlags(&s.loop_ev[0], hipEventDisableTiming));
+  TL_HIP(hipEventCreateWithFlags(&s.loop_ev[1], hipEventDisableTiming));
   TL_HIP(hipStreamSynchronize(s.stream));
   return TL_OK;
 }
@@ -344,7 +353,7 @@ extern "C" int tl_device_bytes(double *bytes) {
   b += (double)s.snaps.size() * (double)s.field_bytes;
   b += (double)s.f32_bytes;
   b += sizeof(double) * (double)MAX_GRID * NPART;
-  if (s.hist) b += sizeof(double) * 3.0 * (double)s.hist_len;
+  if (s.hist) b += sizeof(double) * 4.0 * (double)s.hist_len;
   *bytes = b;
   return TL_OK;
 }

@@ -24,18 +24,25 @@ __device__ inline T op_cell(T c, T l, T r, T d, T u, T kl, T kr, T kd, T ku, T r
 }
 
 // Walk one tile.  For each row yb of the tile (in walking order),
-// body(yb, s1, r1, s0, w, mi) receives stage A's results on that row (sd'
-// and rtemp'), stage A's operand there (sd), A sd' (for stage B) and Mi.
+// body(yb, s1, r1, s0, w, mi, x) receives stage A's results on that row (sd'
+// and rtemp'), stage A's operand there (sd), A sd' (for stage B), Mi (DIAG)
+// and x = xtra(col, yb), the body's own loads of row yb (e.g. utemp).
 // opA(col, row) / opA1(col, row) load stage A's operand (V values / one
 // value), rtA / rtA1 its rtemp (r on the first sweep).  The body must not
 // load from memory written by this pass.
-template <bool NT, typename T, int V, bool DIFF, typename OPA, typename OPA1, typename RTA,
-          typename RTA1, typename BODY>
+// Every load of an iteration (stage A's row, the ghost cells, xtra of stage
+// B's row) is issued before its barrier and arithmetic; with PF stage A's
+// loads are issued one iteration earlier still and xtra two (a row of loads
+// in flight behind the arithmetic, at the cost of the registers that hold
+// it).
+template <bool NT, typename T, int V, bool DIFF, int DIAG, bool PF, typename OPA, typename OPA1,
+          typename RTA, typename RTA1, typename XTRA, typename BODY>
 __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
-                                 const T *__restrict__ Ky, const T *__restrict__ Mi, int diag,
-                                 T rx, T ry, T a1, T b1, OPA opA, OPA1 opA1, RTA rtA, RTA1 rtA1,
+                                 const T *__restrict__ Ky, const T *__restrict__ Mi, T rx, T ry,
+                                 T a1, T b1, OPA opA, OPA1 opA1, RTA rtA, RTA1 rtA1, XTRA xtra,
                                  BODY body, int dir) {
   typedef typename Vec<T, V>::type VT;
+  typedef decltype(xtra(0, 0)) X;
   constexpr int NW = BLOCK / WAVE;
   // per row parity and wave: stage A first and last value, stage A first Kx,
   // stage B first and last value
@@ -66,6 +73,33 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
   // Ky faces behind / ahead of row y in walking direction
   auto face_b = [&](int y) -> long { return (long)(fwd ? y : y + 1) * P; };
   auto face_a = [&](int y) -> long { return (long)(fwd ? y + 1 : y) * P; };
+  // rows of stage B that exist: the tile's rows
+  auto is_b = [&](int y) { return (y - b_first) * dir >= 0 && (b_last - y) * dir >= 0; };
+
+  // the loads of one stage-A row ya (and xtra of the stage-B row it becomes)
+  struct RowIn {
+    VT sa, kx, ka, rt, mi;
+    T ga, gka, gout, gkl, gkr, grt, gmi, ekr;
+    X xb;
+  };
+  auto load_row = [&](int ya, const VT &sc_, T gc_, RowIn &o) {
+    o.sa = in_mesh(ya + dir) ? opA(cl, ya + dir) : sc_;
+    o.kx = ldv<NT, T, V>(Kx + (long)ya * P + cl);
+    o.ka = ldv<NT, T, V>(Ky + face_a(ya) + cl);
+    o.rt = rtA(cl, ya);
+    if (DIAG) o.mi = ldv<NT, T, V>(Mi + (long)ya * P + cl);
+    if (ghost) {
+      o.ga = in_mesh(ya + dir) ? opA1(gx, ya + dir) : gc_;
+      o.gka = Ky[face_a(ya) + gx];
+      const int go = ghost_l ? gx - 1 : gx + 1;  // the ghost's outer neighbour
+      o.gout = (go >= 0 && go < nx) ? opA1(go, ya) : gc_;
+      o.gkl = Kx[(long)ya * P + gx];
+      o.gkr = Kx[(long)ya * P + gx + 1];
+      o.grt = rtA1(gx, ya);
+      if (DIAG) o.gmi = Mi[(long)ya * P + gx];
+    }
+    if (kx_r) o.ekr = Kx[(long)ya * P + xe];
+  };
 
   // stage A state (row ya): operand behind / centre, face behind
   VT sc = opA(cl, a_first);
@@ -78,41 +112,44 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
     gkb = Ky[face_b(a_first) + gx];
   }
   // stage B state (row yb): stage A's results behind / at yb, and the faces,
-  // Mi and operand of stage A at yb
+  // Mi, operand and xtra of stage A at yb
   VT s1b = VT(0), s1c = VT(0), r1c = VT(0), s0c = VT(0);
   VT kxB = VT(0), kbB = VT(0), kaB = VT(0), miB = VT(0);
   T gs1c = 0, kxrB = 0;
+  X xB = X();
+  RowIn nx_in;  // PF: the loads of the next iteration's row, in flight
+  if (PF) {
+    nx_in.ekr = 0;
+    load_row(a_first, sc, gc, nx_in);
+    if (is_b(a_first)) nx_in.xb = xtra(cl, a_first);
+  }
   for (int k = 0; k < iters; ++k) {
     const bool doA = k < m;
     const int ya = a_first + k * dir;
     const int yb = ya - dir;
-    const bool doB = k >= 1 && (yb - b_first) * dir >= 0 && (b_last - yb) * dir >= 0;
-    // ---- loads of stage A's row
-    VT sa = sc, kx = VT(0), ka = kb, rt = VT(0), mi = VT(0);
-    T ga = gc, gka = gkb, gout = gc, gkl = 0, gkr = 0, grt = 0, gmi = 0, ekr = 0;
-    if (doA) {
-      sa = in_mesh(ya + dir) ? opA(cl, ya + dir) : sc;
-      kx = ldv<NT, T, V>(Kx + (long)ya * P + cl);
-      ka = ldv<NT, T, V>(Ky + face_a(ya) + cl);
-      rt = rtA(cl, ya);
-      if (diag) mi = ldv<NT, T, V>(Mi + (long)ya * P + cl);
-      if (ghost) {
-        ga = in_mesh(ya + dir) ? opA1(gx, ya + dir) : gc;
-        gka = Ky[face_a(ya) + gx];
-        const int go = ghost_l ? gx - 1 : gx + 1;  // the ghost's outer neighbour
-        gout = (go >= 0 && go < nx) ? opA1(go, ya) : gc;
-        gkl = Kx[(long)ya * P + gx];
-        gkr = Kx[(long)ya * P + gx + 1];
-        grt = rtA1(gx, ya);
-        if (diag) gmi = Mi[(long)ya * P + gx];
+    const bool doB = k >= 1 && is_b(yb);
+    // ---- loads of stage A's row (issued now, or one iteration ago with PF)
+    RowIn in;
+    in.sa = sc; in.kx = VT(0); in.ka = kb; in.rt = VT(0); in.mi = VT(0);
+    in.ga = gc; in.gka = gkb; in.gout = gc; in.gkl = 0; in.gkr = 0; in.grt = 0; in.gmi = 0;
+    in.ekr = 0;
+    in.xb = X();
+    if (PF) {
+      if (doA) in = nx_in;
+      if (k + 1 < m) {
+        nx_in.ekr = 0;
+        load_row(ya + dir, in.sa, in.ga, nx_in);
+        if (is_b(ya + dir)) nx_in.xb = xtra(cl, ya + dir);
       }
-      if (kx_r) ekr = Kx[(long)ya * P + xe];
+    } else {
+      if (doA) load_row(ya, sc, gc, in);
+      if (doB) xB = xtra(cl, yb);
     }
     // ---- exchange the edge values of both stages between the waves
     const int par = k & 1;
     if (lane == 0) {
       xch[par][wave][0] = sc[0];
-      xch[par][wave][2] = kx[0];
+      xch[par][wave][2] = in.kx[0];
       xch[par][wave][3] = s1c[0];
     }
     if (lane == WAVE - 1) {
@@ -121,7 +158,7 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
     }
     __syncthreads();
     T leftA = __shfl_up(sc[V - 1], 1, WAVE), rightA = __shfl_down(sc[0], 1, WAVE);
-    T kxrA = __shfl_down(kx[0], 1, WAVE);
+    T kxrA = __shfl_down(in.kx[0], 1, WAVE);
     T leftB = __shfl_up(s1c[V - 1], 1, WAVE), rightB = __shfl_down(s1c[0], 1, WAVE);
     if (lane == 0 && wave > 0) {
       leftA = xch[par][wave - 1][1];
@@ -132,7 +169,7 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
       kxrA = xch[par][wave + 1][2];
       rightB = xch[par][wave + 1][3];
     }
-    if (kx_r) kxrA = ekr;
+    if (kx_r) kxrA = in.ekr;
     // ---- stage A on row ya: own columns, then the ghost column
     VT s1a = s1c;  // stage A's result ahead of yb (the mirror when absent)
     VT r1a = r1c;
@@ -141,8 +178,8 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
       if (ghost_l) leftA = gc;
       if (ghost_r) rightA = gc;
       if (c == 0) leftA = sc[0];  // mirror at the left edge
-      const VT &vd = fwd ? sb : sa, &vu = fwd ? sa : sb;
-      const VT &kyd = fwd ? kb : ka, &kyu = fwd ? ka : kb;
+      const VT &vd = fwd ? sb : in.sa, &vu = fwd ? in.sa : sb;
+      const VT &kyd = fwd ? kb : in.ka, &kyu = fwd ? in.ka : kb;
       VT w;
 #pragma unroll
       for (int e = 0; e < V; ++e) {
@@ -151,20 +188,20 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
         T r;
         if (e + 1 < V) r = (col + 1 < nx) ? sc[e + 1] : sc[e];
         else r = (col + 1 < nx) ? rightA : sc[e];
-        const T kr = (e + 1 < V) ? kx[e + 1] : kxrA;
-        w[e] = op_cell<DIFF>(sc[e], l, r, vd[e], vu[e], kx[e], kr, kyd[e], kyu[e], rx, ry);
+        const T kr = (e + 1 < V) ? in.kx[e + 1] : kxrA;
+        w[e] = op_cell<DIFF>(sc[e], l, r, vd[e], vu[e], in.kx[e], kr, kyd[e], kyu[e], rx, ry);
       }
-      r1a = rt - w;
-      s1a = diag ? a1 * sc + (b1 * mi) * r1a : a1 * sc + b1 * r1a;
+      r1a = in.rt - w;
+      s1a = DIAG ? a1 * sc + (b1 * in.mi) * r1a : a1 * sc + b1 * r1a;
       if (ghost) {
         // inner neighbour: the block's edge cell; outer: gout
         const T inner = ghost_l ? sc[0] : sc[V - 1];
-        const T gl = ghost_l ? gout : inner, gr = ghost_l ? inner : gout;
-        const T gd = fwd ? gb : ga, gu = fwd ? ga : gb;
-        const T gkd = fwd ? gkb : gka, gku = fwd ? gka : gkb;
-        const T gw = op_cell<DIFF>(gc, gl, gr, gd, gu, gkl, gkr, gkd, gku, rx, ry);
-        const T grt1 = grt - gw;
-        gs1a = diag ? a1 * gc + (b1 * gmi) * grt1 : a1 * gc + b1 * grt1;
+        const T gl = ghost_l ? in.gout : inner, gr = ghost_l ? inner : in.gout;
+        const T gd = fwd ? gb : in.ga, gu = fwd ? in.ga : gb;
+        const T gkd = fwd ? gkb : in.gka, gku = fwd ? in.gka : gkb;
+        const T gw = op_cell<DIFF>(gc, gl, gr, gd, gu, in.gkl, in.gkr, gkd, gku, rx, ry);
+        const T grt1 = in.grt - gw;
+        gs1a = DIAG ? a1 * gc + (b1 * in.gmi) * grt1 : a1 * gc + b1 * grt1;
       }
     }
     // ---- stage B on row yb
@@ -187,7 +224,7 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
         const T kr = (e + 1 < V) ? kxB[e + 1] : kxrB;
         w[e] = op_cell<DIFF>(s1c[e], l, r, vd[e], vu[e], kxB[e], kr, kyd[e], kyu[e], rx, ry);
       }
-      body(yb, s1c, r1c, s0c, w, miB);
+      body(yb, s1c, r1c, s0c, w, miB, xB);
     }
     // ---- advance: stage A's row becomes stage B's row
     s1b = s1c;
@@ -195,17 +232,18 @@ __device__ inline void walk_pair(const Geo &g, const TileV<V> &tt, const T *__re
     r1c = r1a;
     gs1c = gs1a;
     s0c = sc;
-    kxB = kx;
+    kxB = in.kx;
     kxrB = kxrA;
     kbB = kb;
-    kaB = ka;
-    miB = mi;
+    kaB = in.ka;
+    miB = in.mi;
+    if (PF) xB = in.xb;
     sb = sc;
-    sc = sa;
-    kb = ka;
+    sc = in.sa;
+    kb = in.ka;
     gb = gc;
-    gc = ga;
-    gkb = gka;
+    gc = in.ga;
+    gkb = in.gka;
   }
 }
 

@@ -24,18 +24,35 @@ def host_lib():
         if not os.path.exists(HOST_LIB):
             raise _lib.TeaLeafError("libtealeaf_host.so is not built; run __graft_entry__.build()")
         L = C.CDLL(HOST_LIB, mode=C.RTLD_GLOBAL)
-        L.tea_host_init.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int]
-        L.tea_host_init.restype = C.c_int
-        L.tea_host_step.argtypes = []
-        L.tea_host_step.restype = C.c_int
-        L.tea_host_run.argtypes = []
-        L.tea_host_run.restype = C.c_int
+        for name in ("tea_host_init", "tea_host_guarded_init"):
+            getattr(L, name).argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_int]
+            getattr(L, name).restype = C.c_int
+        for name in ("tea_host_step", "tea_host_run", "tea_host_finalize", "tea_host_guarded_step",
+                     "tea_host_guarded_run", "tea_host_guarded_finalize"):
+            getattr(L, name).argtypes = []
+            getattr(L, name).restype = C.c_int
         L.tea_host_last.argtypes = [C.POINTER(C.c_double)]
         L.tea_host_last.restype = C.c_int
-        L.tea_host_finalize.argtypes = []
-        L.tea_host_finalize.restype = C.c_int
+        L.tea_host_error.argtypes = []
+        L.tea_host_error.restype = C.c_char_p
         _host = L
     return _host
+
+
+# returned by a guarded entry point when the host reported an error (the
+# reference's report_error, report.f90:29-58, which ends the program there)
+HOST_ERROR = -2
+
+
+def _guarded(rc):
+    """Turn a report_error inside a guarded host call into TeaLeafError.  The
+    run is abandoned: the host is finalized so that another can start."""
+    if rc != HOST_ERROR:
+        return rc
+    L = host_lib()
+    msg = L.tea_host_error().decode(errors="replace")
+    L.tea_host_guarded_finalize()
+    raise _lib.TeaLeafError(msg)
 
 
 class TeaHost:
@@ -46,21 +63,25 @@ class TeaHost:
         d = os.fsencode(deck_path)
         o = os.fsencode(out_path)
         self.out_path = out_path
-        rc = L.tea_host_init(d, len(d), o, len(o), device)
+        rc = _guarded(L.tea_host_guarded_init(d, len(d), o, len(o), device))
         if rc != 0:
             raise _lib.TeaLeafError(f"tea_host_init failed ({rc})")
         self.complete = False
 
     def step(self):
-        """One timestep; returns True when the run is complete."""
-        rc = host_lib().tea_host_step()
+        """One timestep; returns True when the run is complete.  A deck or
+        device error raises TeaLeafError and ends the run."""
+        rc = _guarded(host_lib().tea_host_guarded_step())
         if rc < 0:
             raise _lib.TeaLeafError("tea_host_step: host not initialised")
         self.complete = rc == 1
         return self.complete
 
     def run(self):
-        return host_lib().tea_host_run()
+        rc = _guarded(host_lib().tea_host_guarded_run())
+        if rc < 0:
+            raise _lib.TeaLeafError("tea_host_run: host not initialised")
+        return rc
 
     def last(self):
         out = (C.c_double * 14)()
@@ -71,7 +92,7 @@ class TeaHost:
         return dict(zip(keys, list(out)))
 
     def close(self):
-        host_lib().tea_host_finalize()
+        _guarded(host_lib().tea_host_guarded_finalize())
 
     def __enter__(self):
         return self

@@ -11,6 +11,7 @@ MODULE tea_host
 
   USE ISO_C_BINDING
   USE tl_hip_api
+  USE tl_eigen
   IMPLICIT NONE
 
   ! ---- constants (data.f90) -------------------------------------------------
@@ -80,6 +81,16 @@ MODULE tea_host
   CHARACTER(LEN=LINE_LEN + 2) :: cur_line
   CHARACTER(LEN=64) :: here
 
+  INTERFACE
+    !> tea_host_guard.c: hands an error to a guarded caller (does not return
+    !> then), or returns when the host runs as a program
+    SUBROUTINE tea_host_raise(loc, nloc, msg, nmsg) BIND(C, name="tea_host_raise")
+      IMPORT :: c_char, c_int
+      CHARACTER(KIND=c_char), INTENT(IN) :: loc(*), msg(*)
+      INTEGER(c_int), VALUE :: nloc, nmsg
+    END SUBROUTINE tea_host_raise
+  END INTERFACE
+
 CONTAINS
 
   ! ===========================================================================
@@ -98,6 +109,10 @@ CONTAINS
       WRITE(g_out, '(a)') ' TEA is terminating.'
       FLUSH(g_out)
     ENDIF
+    ! inside a guarded entry point (tea_host_guard.c) control returns to the
+    ! caller from here; otherwise the program stops as the reference does
+    CALL tea_host_raise(location, INT(LEN_TRIM(location), c_int), msg, &
+                        INT(LEN_TRIM(msg), c_int))
     ERROR STOP 1
   END SUBROUTINE report_error
 
@@ -258,7 +273,8 @@ CONTAINS
     w = word
     IF (SCAN(w, '.') == 0) w = TRIM(w)//'.0'
     READ(w, '(E27.20)', IOSTAT=ios) get_rval
-    IF (ios /= 0) CALL report_error('parse_getrval', 'Error attempting to convert to real:'//TRIM(word))
+    ! the message shows the word as converted, with any '.0' appended (parse.f90:343-349)
+    IF (ios /= 0) CALL report_error('parse_getrval', 'Error attempting to convert to real:'//TRIM(w))
   END FUNCTION get_rval
 
   !> read_input.f90: defaults, keyword switch, PPCG default, state shrink
@@ -653,115 +669,6 @@ CONTAINS
     WRITE(0, "(' Step ', i7,' time ', f11.7,' timestep  ',1pe9.2,i8)") step, time, dt
   END SUBROUTINE timestep
 
-  ! ---- host scalar routines of tea_leaf_cheby.f90 ---------------------------
-  !> Eigenvalues of the symmetric tridiagonal (d, e) by the implicit QL
-  !> method with Wilkinson-style shifts; e(2:n) holds the sub-diagonal on
-  !> entry.  info = 1 after 30 sweeps without convergence of an eigenvalue.
-  SUBROUTINE ql_eigenvalues(d, e, n, info)
-    INTEGER, INTENT(IN) :: n
-    REAL(KIND=8), INTENT(INOUT) :: d(n), e(n)
-    INTEGER, INTENT(OUT) :: info
-    INTEGER :: l, m, i, sweeps
-    REAL(KIND=8) :: g, r, s, c, p, f, b, tol
-    LOGICAL :: underflow
-    info = 0
-    IF (n < 1) RETURN
-    e(1:n-1) = e(2:n)
-    e(n) = 0.0_8
-    DO l = 1, n
-      sweeps = 0
-      DO
-        ! find the first negligible sub-diagonal at or after l
-        m = l
-        DO WHILE (m < n)
-          tol = ABS(d(m)) + ABS(d(m + 1))
-          IF (ABS(e(m)) + tol == tol) EXIT
-          m = m + 1
-        ENDDO
-        IF (m == l) EXIT
-        IF (sweeps == 30) THEN
-          info = 1
-          RETURN
-        ENDIF
-        sweeps = sweeps + 1
-        g = (d(l + 1) - d(l)) / (2.0_8 * e(l))
-        r = SQRT(g * g + 1.0_8)
-        g = d(m) - d(l) + e(l) / (g + SIGN(r, g))
-        s = 1.0_8
-        c = 1.0_8
-        p = 0.0_8
-        underflow = .FALSE.
-        DO i = m - 1, l, -1
-          f = s * e(i)
-          b = c * e(i)
-          r = SQRT(f * f + g * g)
-          e(i + 1) = r
-          IF (r == 0.0_8) THEN
-            d(i + 1) = d(i + 1) - p
-            e(m) = 0.0_8
-            underflow = .TRUE.
-            EXIT
-          ENDIF
-          s = f / r
-          c = g / r
-          g = d(i + 1) - p
-          r = (d(i) - g) * s + 2.0_8 * c * b
-          p = s * r
-          d(i + 1) = g + p
-          g = c * r - b
-        ENDDO
-        IF (underflow) CYCLE
-        d(l) = d(l) - p
-        e(l) = g
-        e(m) = 0.0_8
-      ENDDO
-    ENDDO
-  END SUBROUTINE ql_eigenvalues
-
-  !> Lanczos tridiagonal from the CG coefficients and its extreme eigenvalues
-  !> (tea_leaf_cheby.f90:142-176).
-  SUBROUTINE calc_eigenvalues(al, be, nsteps, emin, emax, info)
-    INTEGER, INTENT(IN) :: nsteps
-    REAL(KIND=8), INTENT(IN) :: al(0:), be(0:)
-    REAL(KIND=8), INTENT(OUT) :: emin, emax
-    INTEGER, INTENT(OUT) :: info
-    REAL(KIND=8), ALLOCATABLE :: d(:), e(:)
-    INTEGER :: i
-    ALLOCATE(d(MAX(nsteps, 1)), e(MAX(nsteps, 1)))
-    d = 0.0_8
-    e = 0.0_8
-    DO i = 1, nsteps
-      d(i) = 1.0_8 / al(i)
-      IF (i > 1) d(i) = d(i) + be(i - 1) / al(i - 1)
-      IF (i < nsteps) e(i + 1) = SQRT(be(i)) / al(i)
-    ENDDO
-    CALL ql_eigenvalues(d, e, nsteps, info)
-    emin = 0.0_8
-    emax = 0.0_8
-    IF (info /= 0) RETURN
-    emin = MINVAL(d(1:nsteps))
-    emax = MAXVAL(d(1:nsteps))
-    IF (emin < 0.0_8 .OR. emax < 0.0_8) info = 1
-  END SUBROUTINE calc_eigenvalues
-
-  !> Chebyshev recurrence coefficients (tea_leaf_cheby.f90:178-205)
-  SUBROUTINE calc_ch_coefs(ca, cb, emin, emax, th, n)
-    INTEGER, INTENT(IN) :: n
-    REAL(KIND=8), INTENT(OUT) :: ca(:), cb(:), th
-    REAL(KIND=8), INTENT(IN) :: emin, emax
-    REAL(KIND=8) :: delta, sigma, rho_old, rho_new
-    INTEGER :: i
-    th = (emax + emin) / 2
-    delta = (emax - emin) / 2
-    sigma = th / delta
-    rho_old = 1.0_8 / sigma
-    DO i = 1, n
-      rho_new = 1.0_8 / (2.0_8 * sigma - rho_old)
-      ca(i) = rho_new * rho_old
-      cb(i) = 2.0_8 * rho_new / delta
-      rho_old = rho_new
-    ENDDO
-  END SUBROUTINE calc_ch_coefs
 
   ! ===========================================================================
   ! tea_leaf(): one implicit solve (tea_solve.f90:37-503).  The iteration loop
@@ -957,6 +864,7 @@ CONTAINS
               IF (k == m .AND. is_check) error = norm
               cheby_calc_steps = cheby_calc_steps + 1
               error = SQRT(ABS(error))
+              IF (verbose_on) WRITE(g_out, *) "Residual ", error
               IF (ABS(error) < eps * initial_residual) THEN
                 n = k
                 exited = .TRUE.
@@ -978,6 +886,7 @@ CONTAINS
           inner_c = INT(ppcg_inner_iters, c_int)
           CALL hip_check(tl_ppcg_loop(st, ch_alphas, ch_betas, INT(tl_ppcg_inner_steps, c_int), &
                                       theta, inner_c), 'tea_leaf_ppcg')
+          CALL print_residuals(n, st%n_end)
           ppcg_inner_iters = inner_c
           cheby_calc_steps = cheby_calc_steps + (st%n_end - n + 1)
           rro = st%rro
@@ -1004,6 +913,8 @@ CONTAINS
         st%rro = rro
         st%old_error = old_error
         CALL hip_check(tl_cg_loop(st, cg_alphas, cg_betas), 'tea_leaf_cg')
+        ! a switch is found at the top of n_end: n_end - 1 was the last CG iteration
+        CALL print_residuals(n, st%n_end - MERGE(1, 0, st%reason == TL_LOOP_SWITCH))
         ! CG iterations run (:343); a switch is found at the top of n_end
         cg_calc_steps = cg_calc_steps + st%n_end - st%n_first + MERGE(0, 1, st%reason == TL_LOOP_SWITCH)
         rro = st%rro
@@ -1027,6 +938,7 @@ CONTAINS
         st%eps_abs = eps * initial_residual
         st%check_switch = 0
         CALL hip_check(tl_jacobi_loop(st), 'tea_leaf_jacobi')
+        CALL print_residuals(n, st%n_end)
         error = st%error
         IF (st%reason == TL_LOOP_CONVERGED) THEN
           n = st%n_end
@@ -1111,6 +1023,22 @@ CONTAINS
     ENDIF
     DEALLOCATE(cg_alphas, cg_betas, ch_alphas, ch_betas)
   END SUBROUTINE tea_leaf
+
+  !> verbose_on: the "Residual" line the reference prints after every
+  !> iteration (tea_solve.f90:402-408), for iterations lo..hi of the last
+  !> fused loop, from the errors the device kept (tl_loop_errors)
+  SUBROUTINE print_residuals(lo, hi)
+    INTEGER, INTENT(IN) :: lo, hi
+    REAL(KIND=8), ALLOCATABLE :: e(:)
+    INTEGER :: k
+    IF (.NOT. verbose_on .OR. hi < lo) RETURN
+    ALLOCATE(e(0:hi))
+    CALL hip_check(tl_loop_errors(INT(lo, c_int), INT(hi, c_int), e), 'tea_leaf')
+    DO k = lo, hi
+      WRITE(g_out, *) "Residual ", e(k)
+    ENDDO
+    DEALLOCATE(e)
+  END SUBROUTINE print_residuals
 
   !> Mixed-precision solve of the step (tl_mixed_solve): FP32 CG or PPCG
   !> inner solves under FP64 refinement until the true residual has dropped

@@ -7,7 +7,8 @@ MODULE tl_hip_api
   IMPLICIT NONE
 
   INTEGER(c_int), PARAMETER :: TL_OK = 0
-  INTEGER(c_int), PARAMETER :: TL_LOOP_CONVERGED = 1, TL_LOOP_SWITCH = 2, TL_LOOP_MAXITERS = 3
+  INTEGER(c_int), PARAMETER :: TL_LOOP_CONVERGED = 1, TL_LOOP_SWITCH = 2, TL_LOOP_MAXITERS = 3, &
+                                TL_LOOP_NONFINITE = 4, TL_LOOP_FLOOR = 5
 
   INTEGER(c_int), PARAMETER :: TLF_DENSITY = 1, TLF_ENERGY0 = 2, TLF_ENERGY1 = 3, TLF_U = 4, &
                                TLF_P = 5, TLF_SD = 6, TLF_R = 7, TLF_Z = 8, TLF_KX = 9,     &
@@ -160,6 +161,39 @@ MODULE tl_hip_api
     INTEGER(c_int) FUNCTION tl_jacobi_loop(st) BIND(C, name="tl_jacobi_loop")
       IMPORT :: c_int, tl_loop_state
       TYPE(tl_loop_state), INTENT(INOUT) :: st
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_cg_calc_w(pw) BIND(C, name="tl_cg_calc_w")
+      IMPORT :: c_int, c_double
+      REAL(c_double), INTENT(OUT) :: pw
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_cg_calc_ur(alpha, rrn) BIND(C, name="tl_cg_calc_ur")
+      IMPORT :: c_int, c_double
+      REAL(c_double), VALUE :: alpha
+      REAL(c_double), INTENT(OUT) :: rrn
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_cg_calc_p(beta) BIND(C, name="tl_cg_calc_p")
+      IMPORT :: c_int, c_double
+      REAL(c_double), VALUE :: beta
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_ppcg_store_r() BIND(C, name="tl_ppcg_store_r")
+      IMPORT :: c_int
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_ppcg_calc_zrnorm(rro) BIND(C, name="tl_ppcg_calc_zrnorm")
+      IMPORT :: c_int, c_double
+      REAL(c_double), INTENT(OUT) :: rro
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_ppcg_calc_rrn(rrn) BIND(C, name="tl_ppcg_calc_rrn")
+      IMPORT :: c_int, c_double
+      REAL(c_double), INTENT(OUT) :: rrn
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_jacobi_solve(err) BIND(C, name="tl_jacobi_solve")
+      IMPORT :: c_int, c_double
+      REAL(c_double), INTENT(OUT) :: err
+    END FUNCTION
+    INTEGER(c_int) FUNCTION tl_loop_errors(lo, hi, err) BIND(C, name="tl_loop_errors")
+      IMPORT :: c_int, c_double
+      INTEGER(c_int), VALUE :: lo, hi
+      REAL(c_double), INTENT(INOUT) :: err(*)
     END FUNCTION
   END INTERFACE
 

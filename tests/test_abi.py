@@ -174,6 +174,7 @@ def test_fortran_struct_layouts_match(tmp_path):
 def test_host_library_exports():
     L = host.host_lib()
     for n in ("tea_host_init", "tea_host_step", "tea_host_run", "tea_host_last",
-              "tea_host_finalize"):
+              "tea_host_finalize", "tea_host_guarded_init", "tea_host_guarded_step",
+              "tea_host_guarded_run", "tea_host_guarded_finalize", "tea_host_error"):
         assert hasattr(L, n)
     assert os.access(host.TEA_LEAF_BIN, os.X_OK)

@@ -1,6 +1,11 @@
 """End to end: the native host (Fortran, in process) runs each reference deck on
 the GPU; its tea.out must reproduce the reference's per-step `step:` rows
-within 1e-10 relative and its iteration counts (tea_solve.f90:448-453)."""
+within 1e-10 relative and its iteration counts (tea_solve.f90:448-453):
+exactly on 119 of the 132 steps the fixtures hold.  The other 13 are exits
+decided by rounding (fixtures_util.iteration_bounds): the reference and the
+oracle built with multiply-add contraction disagree there, or, at tl_eps
+1e-15, the reference's exit error lies within 2 % of the threshold; there
+the count may differ by one.  The per-step differences are printed."""
 
 import pytest
 
@@ -21,7 +26,7 @@ def test_deck_matches_reference(name, tmp_path):
     print(f"[e2e] {name}", flush=True)
     got = host.run_deck(fx["deck"], str(tmp_path))
     FX.compare_summaries(got["summaries"], fx["summaries"], RTOL)
-    FX.compare_iterations(got["steps"], fx["steps"], slack=FX.iteration_slack(fx["deck"]))
+    FX.compare_iterations_exact(got["steps"], fx, name)
     for k, r in fx["steps"].items():
         g = got["steps"][int(k)]
         if "ppcg_inner" in r:
