@@ -159,7 +159,7 @@ CLASS_TRAFFIC = {0: ("k_loop_w", "cg_fp64_4096_hbm_traffic.md"),
                  6: ("k_pp_ur", "ppcg_fp64_4096_hbm_traffic.md")}
 
 
-def work_launches(solver, rows, inner, max_iters=10000):
+def work_launches(solver, rows, inner=0, max_iters=10000):
     """Launches per kernel class that did work in the steps of `rows`
     (TeaHost.last()), from the iteration counts: CG iterations before the
     switch (two passes each), then Chebyshev iterations two per pass, PPCG
@@ -388,41 +388,62 @@ def host_true_residual(cells, rx, ry):
 
 # tl_eps of the FP64 forms in the precision comparison: the largest of 1e-10,
 # 1e-11, ... at which their exit (the reference's recurrence test) leaves a
-# true relative residual <= 1e-10 (measured on this deck at 4096^2: CG's
-# recurrence residual tracks the true one, 9.99e-11 at 1e-10; PPCG's exit
-# quantity (r - r_store).z does not: 1.5e-8 at 1e-10, 1.5e-10 at 1e-12,
-# 1.5e-11 at 1e-13); the mixed forms stop on the true residual itself
+# true relative residual <= 1e-10 on the first step (measured on this deck at
+# 4096^2: CG's recurrence residual tracks the true one, 9.99e-11 at 1e-10;
+# PPCG's exit quantity (r - r_store).z does not: 1.5e-8 at 1e-10, 1.5e-10 at
+# 1e-12, 1.5e-11 at 1e-13); the mixed forms stop on the true residual itself
 PRECISION_EPS = {"fp64_cg": 1e-10, "fp64_ppcg": 1e-13, "mixed_cg": 1e-10, "mixed_ppcg": 1e-10}
 
+# the FP64 forms' iteration cap in the comparison: 3x the CG count at 8192^2
+PRECISION_MAX_ITERS = 40000
 
-def precision_comparison(cells, steps, warmup, inner_steps=10, target=1e-10):
+
+def precision_comparison(cells, inner_steps=10, target=1e-10, later_step=False):
     """FP64 CG, FP64 PPCG and the FP32-inner / FP64-refined CG and PPCG on the
-    inverse-density deck, each to a true relative residual of `target`.  The
-    FP64 forms stop on the reference's recurrence test (tea_solve.f90:410), at
-    the tl_eps of PRECISION_EPS that brings their true residual to the target;
+    inverse-density deck, each to a true relative residual of `target`, timed
+    on the deck's first step (every form starts from the same u0).  The FP64
+    forms stop on the reference's recurrence test (tea_solve.f90:410), at the
+    tl_eps of PRECISION_EPS that brings their true residual to the target;
     the mixed forms stop on the true residual.  Every form's result is then
     measured the same way (host_true_residual).  Errors of u and of the
-    summary are relative to the FP64 CG result."""
+    summary are relative to the FP64 CG result.
+
+    With later_step, FP64 CG and mixed PPCG also run the second step (after
+    an untimed first): there the starting residual u0 - A u0 is smaller than
+    on the first step while the rounding of the reference's FP64 product A p
+    (scaled by |A| |u|) is not, so its true residual can level off above
+    the target; the mixed form refines on the difference-form residual,
+    whose rounding scales with the fluxes.  Each record reports what the form
+    reached."""
     from tealeaf_ref_amd.deckgen import bench_deck
     import numpy as np
     runs = {}
-    base = dict(coefficient="inverse_density", max_iters=100000, end_step=steps + warmup,
-                summary_frequency=1)
     plan = [("fp64_cg", "cg", ()), ("fp64_ppcg", "ppcg", ()),
             ("mixed_cg", "cg", ("tl_mixed_precision",)),
             ("mixed_ppcg", "ppcg", ("tl_mixed_precision",))]
     ref_u = ref_sum = None
     dx = 10.0 / cells
     rx = 0.004 / (dx * dx)
-    for name, solver, extra in plan:
-        kw = dict(base, extra=extra, eps=PRECISION_EPS[name])
+
+    def deck_of(name, solver, extra, end_step, cells=cells):
+        kw = dict(coefficient="inverse_density", max_iters=PRECISION_MAX_ITERS,
+                  end_step=end_step, summary_frequency=1, extra=extra, eps=PRECISION_EPS[name])
         if solver == "ppcg":
             kw["inner_steps"] = inner_steps
-        deck = bench_deck(cells, solver=solver, **kw)
+        return bench_deck(cells, solver=solver, **kw)
+
+    for name, solver, extra in plan:
+        # one untimed step of the same form on a small mesh first, so that no
+        # form pays the library's first-launch costs
+        with tempfile.TemporaryDirectory() as wd:
+            run_deck_timed(deck_of(name, solver, extra, 1, cells=256), 1, 0, wd)
+        log(f"[bench] {cells}^2 {name}: start")
         prof = {} if name == "mixed_ppcg" else None
         with tempfile.TemporaryDirectory() as wd:
-            elapsed, rows, u, tres = run_deck_timed(deck, steps, warmup, wd, want_u=True,
-                                                    prof_out=prof, residual=(rx, rx))
+            elapsed, rows, u, tres = run_deck_timed(deck_of(name, solver, extra, 1), 1, 0, wd,
+                                                    want_u=True, prof_out=prof,
+                                                    residual=(rx, rx))
+        steps = 1
         last = rows[-1]
         rec = {"tl_eps": PRECISION_EPS[name], "s_per_step": elapsed / steps,
                "iterations_per_step": sum(r["iterations"] for r in rows) / steps,
@@ -450,9 +471,25 @@ def precision_comparison(cells, steps, warmup, inner_steps=10, target=1e-10):
         runs[name] = rec
     out = {"workload": f"{cells}x{cells}, seeded deck (3 rectangles + 2 circles, seed 2), "
                        f"tl_coefficient_inverse_density, true relative residual {target:g}, "
-                       f"PPCG {inner_steps} inner steps, {steps} timed steps after {warmup}",
+                       f"PPCG {inner_steps} inner steps, first step timed",
            "runs": runs,
            "all_meet_target": all(r["meets_target"] for r in runs.values())}
+    if later_step:
+        out["second_step"] = {}
+        for name, solver, extra in (plan[0], plan[3]):
+            log(f"[bench] {cells}^2 {name} second step: start")
+            with tempfile.TemporaryDirectory() as wd:
+                elapsed, rows, _, tres = run_deck_timed(deck_of(name, solver, extra, 2), 1, 1,
+                                                        wd, residual=(rx, rx))
+            r = rows[-1]
+            rec = {"tl_eps": PRECISION_EPS[name], "s_per_step": elapsed,
+                   "iterations": r["iterations"], "true_rel_residual": tres,
+                   "meets_target": tres <= target, "solver_exit_residual": r["conduction_error"]}
+            if extra:
+                rec["refinements"] = r["refinements"]
+                rec["sweeps"] = r["ppcg_inner"]
+            log(f"[bench] {cells}^2 {name} second step: {rec}")
+            out["second_step"][name] = rec
     t = {k: v["s_per_step"] for k, v in runs.items()}
     out["speedup_mixed_ppcg_vs_fp64_cg"] = t["fp64_cg"] / t["mixed_ppcg"]
     out["speedup_mixed_cg_vs_fp64_cg"] = t["fp64_cg"] / t["mixed_cg"]
@@ -774,8 +811,6 @@ def main():
                     help="skip the STREAM-style copy/triad bandwidth measurement")
     ap.add_argument("--solver-steps", type=int, default=1,
                     help="timed steps of the Chebyshev and PPCG runs (0 to skip)")
-    ap.add_argument("--mixed-steps", type=int, default=2)
-    ap.add_argument("--mixed-warmup", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the solver keeps the whole mesh as one chunk "
@@ -843,10 +878,8 @@ def main():
             w_gbs / out["achievable_hbm"]["triad_gbs"] if w_gbs else None)
     sizes = [int(x) for x in args.mixed_sizes.split(",") if x.strip()]
     if sizes:
-        out["precision_comparison"] = [
-            precision_comparison(n, *((args.mixed_steps, args.mixed_warmup) if n < 8192
-                                      else (1, 0)))  # 8192^2: the first step, once
-            for n in sizes]
+        out["precision_comparison"] = [precision_comparison(n, later_step=(i == 0))
+                                       for i, n in enumerate(sizes)]
     if args.adjoint_cells:
         out["adjoint"] = adjoint_bench(args.adjoint_cells, args.adjoint_steps, args.adjoint_every,
                                        args.adjoint_fd, args.adjoint_h, args.adjoint_precision,

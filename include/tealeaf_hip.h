@@ -269,8 +269,10 @@ typedef struct {
   double inner_tol;       /* residual reduction asked of an inner solve; one
                            * that needs less to reach rel_tol is asked for a
                            * quarter of what remains                             */
-  /* in/out: PPCG eigenvalue bounds; > 0 on entry reuses them (the matrix is
-   * the same every step), else they are estimated and returned */
+  /* in/out: PPCG eigenvalue bounds; > 0 on entry reuses them, else they are
+   * estimated and returned.  Pass back only bounds of an earlier step of the
+   * same chunk, mesh and rx, ry (the matrix is then the same): bounds below
+   * the top of the spectrum make the polynomial preconditioner indefinite */
   double eigmin, eigmax;
   /* out */
   int outer;              /* refinements done                                    */
@@ -279,6 +281,10 @@ typedef struct {
   int reason;             /* TL_LOOP_CONVERGED, TL_LOOP_FLOOR or TL_LOOP_MAXITERS */
   double initial_residual;/* ||u0 - A u|| at entry                               */
   double final_residual;  /* ||u0 - A u|| on exit (FP64)                         */
+  int eig_iters;          /* PPCG: FP32 CG iterations whose Lanczos tridiagonal
+                           * gave the eigenvalue bounds (presteps, doubled until
+                           * the largest estimate moves by under 1 %); 0 when
+                           * the bounds were passed in                           */
 } tl_mixed_state;
 
 int tl_mixed_solve(tl_mixed_state *st);

@@ -51,7 +51,7 @@ class MixedState(C.Structure):
                 ("inner_tol", C.c_double), ("eigmin", C.c_double), ("eigmax", C.c_double),
                 ("outer", C.c_int), ("inner_iters", C.c_int), ("sweeps", C.c_int),
                 ("reason", C.c_int), ("initial_residual", C.c_double),
-                ("final_residual", C.c_double)]
+                ("final_residual", C.c_double), ("eig_iters", C.c_int)]
 
 
 _lib = None

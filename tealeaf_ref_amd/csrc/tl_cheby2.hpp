@@ -28,11 +28,11 @@ struct ChebyIn {
 // / opA1(col, row) load u (V values / one value); side(col, row) /
 // side1(col, row) load u0, p and Mi (ChebyIn).  The body must not load from
 // memory written by this pass.
-template <bool NT, int V, typename OPA, typename OPA1, typename SIDE, typename SIDE1,
+template <bool NT, int V, int DIAG, typename OPA, typename OPA1, typename SIDE, typename SIDE1,
           typename BODY>
 __device__ inline void walk_cheby_pair(const Geo &g, const TileV<V> &tt,
                                        const double *__restrict__ Kx,
-                                       const double *__restrict__ Ky, int diag, double rx,
+                                       const double *__restrict__ Ky, double rx,
                                        double ry, double a1, double b1, OPA opA, OPA1 opA1,
                                        SIDE side, SIDE1 side1, BODY body, int dir) {
   typedef double T;
@@ -156,7 +156,7 @@ __device__ inline void walk_cheby_pair(const Geo &g, const TileV<V> &tt,
         w[e] = op_cell<false>(sc[e], l, r, vd[e], vu[e], kx[e], kr, kyd[e], kyu[e], rx, ry);
       }
       const VT ra = in.u0 - w;
-      p1a = diag ? a1 * in.p + b1 * (in.mi * ra) : a1 * in.p + b1 * ra;
+      p1a = DIAG ? a1 * in.p + b1 * (in.mi * ra) : a1 * in.p + b1 * ra;
       u1a = sc + p1a;
       if (ghost) {
         // inner neighbour: the block's edge cell; outer: gout
@@ -166,7 +166,7 @@ __device__ inline void walk_cheby_pair(const Geo &g, const TileV<V> &tt,
         const T gkd = fwd ? gkb : gka, gku = fwd ? gka : gkb;
         const T gw = op_cell<false>(gc, gl, gr, gd, gu, gkl, gkr, gkd, gku, rx, ry);
         const T gra = gin.u0 - gw;
-        const T gp1 = diag ? a1 * gin.p + b1 * (gin.mi * gra) : a1 * gin.p + b1 * gra;
+        const T gp1 = DIAG ? a1 * gin.p + b1 * (gin.mi * gra) : a1 * gin.p + b1 * gra;
         gu1a = gc + gp1;
       }
     }

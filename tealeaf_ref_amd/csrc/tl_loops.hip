@@ -356,14 +356,18 @@ __global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_v(Geo G, const 
 #define TL_CHEBY_PAIR_WAVES 3
 #endif
 
-template <bool NT>
+// DIAG (Jacobi-diagonal preconditioner) and LAST (the last pass of a run,
+// which stores w, r, z and the r.r partials) are template parameters, so
+// that the common pass holds no state it does not use.
+template <bool NT, int DIAG, int LAST>
 __global__ __launch_bounds__(BLOCK, TL_CHEBY_PAIR_WAVES) void k_cheby_pair(
     Geo G, const double *__restrict__ uin, double *__restrict__ uout,
     const double *__restrict__ u0, const double *__restrict__ Kx, const double *__restrict__ Ky,
     const double *__restrict__ pin, double *__restrict__ pout, double *__restrict__ w,
     double *__restrict__ r, double *__restrict__ z, const double *__restrict__ Mi, double rx,
-    double ry, double a1, double b1, double a2, double b2, int diag, int last,
-    double *__restrict__ out_part, int dir) {
+    double ry, double a1, double b1, double a2, double b2, double *__restrict__ out_part,
+    int dir) {
+  constexpr bool diag = DIAG != 0, last = LAST != 0;
   typedef typename dev::Vec<double, 2>::type VT;
   __shared__ double lds[BLOCK / WAVE];
   const long P = G.pitch;
@@ -391,8 +395,8 @@ __global__ __launch_bounds__(BLOCK, TL_CHEBY_PAIR_WAVES) void k_cheby_pair(
       e.mi = diag ? Mi[i] : 0.0;
       return e;
     };
-    dev::walk_cheby_pair<NT, 2>(
-        G, tt, Kx, Ky, diag, rx, ry, a1, b1, ldop, ld1, side, side1,
+    dev::walk_cheby_pair<NT, 2, DIAG>(
+        G, tt, Kx, Ky, rx, ry, a1, b1, ldop, ld1, side, side1,
         [&](int y, VT u1, VT p1, VT wv, const dev::ChebyIn<VT> &e) {
       if (tt.c >= G.nx) return;
       const long row = (long)y * P;
@@ -440,7 +444,12 @@ int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_fi
   const int diag = s.precond == TL_PREC_JAC_DIAG;
   const bool nt = use_nontemporal();
   auto k = nt ? k_cheby_v<true> : k_cheby_v<false>;
-  auto k2 = nt ? k_cheby_pair<true> : k_cheby_pair<false>;
+  using K2 = decltype(&k_cheby_pair<true, 0, 0>);
+  static const K2 k2tab[2][2][2] = {
+      {{&k_cheby_pair<false, 0, 0>, &k_cheby_pair<false, 0, 1>},
+       {&k_cheby_pair<false, 1, 0>, &k_cheby_pair<false, 1, 1>}},
+      {{&k_cheby_pair<true, 0, 0>, &k_cheby_pair<true, 0, 1>},
+       {&k_cheby_pair<true, 1, 0>, &k_cheby_pair<true, 1, 1>}}};
   const bool pairs = sweep_pairs();
   for (int i = 0; i < count;) {
     const int st = step_first + i - 1;
@@ -450,11 +459,11 @@ int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_fi
     if (pairs && i + 1 < count) {
       // two iterations per pass; p moves to the other buffer
       const int last = i + 2 == count;
-      hipLaunchKernelGGL(k2, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
-                         s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.p[1 - s.pcur].p11,
-                         s.w.p11, s.r.p11, s.z.p11, s.Mi.p11, s.rx, s.ry, ch_alphas[st],
-                         ch_betas[st], ch_alphas[st + 1], ch_betas[st + 1], diag, last, part(P0),
-                         s.walk_dir);
+      hipLaunchKernelGGL(k2tab[nt ? 1 : 0][diag][last], dim3(nb), dim3(BLOCK), 0, s.stream, G,
+                         s.u[in].p11, s.u[outb].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
+                         s.p[s.pcur].p11, s.p[1 - s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
+                         s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], ch_alphas[st + 1],
+                         ch_betas[st + 1], part(P0), s.walk_dir);
       s.pcur = 1 - s.pcur;
       i += 2;
     } else {

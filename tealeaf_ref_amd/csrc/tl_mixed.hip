@@ -15,6 +15,7 @@
 // Both levels apply the operator in difference form (tl_stencilv.hpp): with
 // Di x - sum K x_n, FP32 rounding limits each refinement to a residual
 // reduction of about cond(A) * 2^-24, which is 1e-2 for the 4096^2 decks.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -344,6 +345,7 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
   st->outer = 0;
   st->inner_iters = 0;
   st->sweeps = 0;
+  st->eig_iters = 0;
   double rr64, rr32;
   if ((rc = refine(false, &rr64, &rr32))) return rc;
   const double r0 = sqrt(rr64);
@@ -369,24 +371,43 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
     rel = rel > st->inner_tol ? rel : st->inner_tol;
     rel = rel < 0.5 ? rel : 0.5;
     if (st->solver == TL_MIXED_CG || !have_eig) {
-      // FP32 CG from d = 0, p = r; for PPCG the first solve runs `presteps`
-      // iterations whose coefficients give the eigenvalue estimate
+      // FP32 CG from d = 0, p = r.  For PPCG this first solve's coefficients
+      // give the eigenvalue estimate (the Lanczos tridiagonal).  Its largest
+      // eigenvalue converges from below, and the x1.05 margin of
+      // tea_solve.f90:193-194 must cover what is left (a bound below the
+      // spectrum makes the polynomial preconditioner amplify the top modes),
+      // so the CG runs in doubling segments from `presteps` iterations until
+      // the largest estimate moved by under 1 % over the last segment, at
+      // most 64 x presteps (or max_inner) iterations in all.
       const bool estimate = st->solver == TL_MIXED_PPCG;
-      const int n_last = estimate ? st->presteps : st->max_inner;
-      std::vector<double> al(n_last + 2, 0.0), be(n_last + 2, 0.0);
+      const int cap = estimate ? std::min(st->max_inner, 64 * st->presteps) : st->max_inner;
+      std::vector<double> al(estimate ? cap + 2 : 0, 0.0), be(estimate ? cap + 2 : 0, 0.0);
       tl_loop_state ls;
       std::memset(&ls, 0, sizeof(ls));
       ls.n_first = 1;
-      ls.n_last = n_last;
+      ls.n_last = estimate ? std::min(st->presteps, cap) : cap;
       ls.eps_abs = rel * sqrt(rr32);
       ls.rro = rr32;
-      if ((rc = run_cg_loop<float, 4>(b, &ls, estimate ? al.data() : nullptr,
-                                      estimate ? be.data() : nullptr, b.r)))
-        return rc;
-      st->inner_iters += ls.n_end;
+      double emin = 0.0, emax = 0.0, emax_prev = 0.0;
+      bool eig_ok = false;
+      for (;;) {
+        // the first segment starts from p = r; later ones continue the loop
+        if ((rc = run_cg_loop<float, 4>(b, &ls, estimate ? al.data() : nullptr,
+                                        estimate ? be.data() : nullptr,
+                                        ls.n_first == 1 ? b.r : nullptr)))
+          return rc;
+        st->inner_iters += ls.n_end - ls.n_first + 1;
+        if (!estimate) break;
+        eig_ok = ls.n_end >= 2 && lanczos_bounds(al, be, ls.n_end, &emin, &emax);
+        st->eig_iters = ls.n_end;
+        if (!eig_ok || ls.reason != TL_LOOP_MAXITERS || ls.n_end >= cap) break;
+        if (emax_prev > 0.0 && emax - emax_prev <= 0.01 * emax) break;
+        emax_prev = emax;
+        ls.n_first = ls.n_end + 1;
+        ls.n_last = std::min(2 * ls.n_end, cap);
+      }
       if (estimate) {
-        double emin, emax;
-        if (ls.n_end >= 2 && lanczos_bounds(al, be, ls.n_end, &emin, &emax)) {
+        if (eig_ok) {
           st->eigmin = emin * 0.95;  // tea_solve.f90:193-194
           st->eigmax = emax * 1.05;
           cheby_coefs(st->eigmin, st->eigmax, st->inner_steps, ca, cb, &theta);

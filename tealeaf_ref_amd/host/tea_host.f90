@@ -1070,7 +1070,12 @@ CONTAINS
     last_refinements = ms%outer
     WRITE(g_out, "('Mixed precision refinements',i6,' initial residual',e14.6,' true residual',e14.6)") &
       ms%outer, initial_residual, ms%final_residual
-    IF (tl_use_ppcg) WRITE(g_out, "('Eigen min',e14.6,' Eigen max',e14.6)") ms%eigmin, ms%eigmax
+    IF (tl_use_ppcg .AND. ms%eig_iters > 0) THEN
+      WRITE(g_out, "('Eigen min',e14.6,' Eigen max',e14.6,' from',i7,' FP32 CG iterations')") &
+        ms%eigmin, ms%eigmax, ms%eig_iters
+    ELSE IF (tl_use_ppcg) THEN
+      WRITE(g_out, "('Eigen min',e14.6,' Eigen max',e14.6)") ms%eigmin, ms%eigmax
+    ENDIF
     IF (.NOT. converged) WRITE(g_out, "(a)") ' Mixed precision solve stopped at the refinement limit'
   END SUBROUTINE mixed_solve
 
@@ -1137,6 +1142,22 @@ CONTAINS
     eigmax = 0.0_8
     theta = 0.0_8
     cn = 0.0_8
+    ! a new problem: no eigenvalue bounds or results carried over from an
+    ! earlier run in this process
+    mixed_eigmin = 0.0_8
+    mixed_eigmax = 0.0_8
+    last_iterations = 0
+    last_ppcg_inner = 0
+    last_switch = 0
+    last_est_itc = 0
+    last_refinements = 0
+    last_cond_error = 0.0_8
+    last_exact_error = 0.0_8
+    last_solve_time = 0.0_8
+    last_summary = 0.0_8
+    last_summary_step = -1
+    first_step_t = 0.0_8
+    second_step_t = 0.0_8
     profiler = profiler_type()
 
     WRITE(g_out, *) 'Setting up initial geometry'

@@ -38,6 +38,7 @@ MODULE tl_hip_api
     REAL(c_double) :: rel_tol, inner_tol, eigmin, eigmax
     INTEGER(c_int) :: outer, inner_iters, sweeps, reason
     REAL(c_double) :: initial_residual, final_residual
+    INTEGER(c_int) :: eig_iters
   END TYPE tl_mixed_state
 
   INTERFACE

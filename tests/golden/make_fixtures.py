@@ -77,6 +77,12 @@ CASES = {
     "layout_verbose_jacobi_24": dict(cells=24, solver="jacobi", seed=18, eps=1e-6,
                                      max_iters=4000, end_step=2, profiler=True,
                                      extra=("verbose_on",)),
+    # larger meshes: the tiling of the two-per-pass kernels over many tiles
+    # and tile bands, end to end against the reference
+    "cheby_large_1024": dict(cells=1024, solver="chebyshev", seed=19, n_circ=2, presteps=20,
+                             eps=1e-10, max_iters=3000, end_step=2),
+    "ppcg_large_1024": dict(cells=1024, solver="ppcg", seed=20, n_circ=2, inner_steps=10,
+                            eps=1e-10, max_iters=3000, end_step=2),
     # visit output: the VTK files and tea.visit are kept too
     "visit_cg_16x12": dict(cells=16, y_cells=12, solver="cg", seed=15, end_step=3,
                            extra=("visit_frequency=2",)),

@@ -43,3 +43,30 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_work_launches_from_iteration_counts():
+    """Launches that did work per kernel class, from a step's printed
+    counts: n-1 printed at an exit at iteration n, max_iters at the cap."""
+    cg = bench.work_launches("cg", [{"iterations": 99, "switch_n": 0, "ppcg_inner": 0}])
+    assert cg[0] == cg[1] == 100 and cg[2] == 0
+    # PPCG: CG up to the switch at the top of iteration 26 (25 CG iterations),
+    # then outer iterations 26..350, five two-sweep passes each for 10 sweeps
+    pp = bench.work_launches("ppcg", [{"iterations": 349, "switch_n": 26, "ppcg_inner": 3260}],
+                             inner=10)
+    assert (pp[0], pp[1]) == (25, 25)
+    assert pp[5] == pp[6] == 325 and pp[2] == 325 * 5
+    # Jacobi at the cap: 10000 iterations, two per pass
+    jl = bench.work_launches("jacobi", [{"iterations": 10000, "switch_n": 0, "ppcg_inner": 0}])
+    assert jl[4] == 5000
+    # Chebyshev: 20 CG iterations, the per-routine first step, then pairs
+    ch = bench.work_launches("chebyshev", [{"iterations": 10000, "switch_n": 21,
+                                            "ppcg_inner": 0}])
+    assert (ch[0], ch[3]) == (20, (10000 - 20 - 1) // 2)
+
+
+def test_cpu_info_reports_cores():
+    info = bench.cpu_info()
+    assert info["nproc"] >= 1 and info["affinity_cpus"] >= 1
+    if "physical_cores" in info:
+        assert 1 <= info["physical_cores"] <= info["nproc"]
