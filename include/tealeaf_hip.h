@@ -218,6 +218,18 @@ typedef struct {
  * cg_alphas(n)/cg_betas(n) at :353,363.  Either may be NULL. */
 int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas);
 
+/* tl_cg_loop without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the loop's control
+ * block, per-iteration histories and partial sums live in a file-scope
+ * device array of 622,976 bytes that each device holds until the process
+ * ends, instead of the library's loop buffers.  Same kernels and results as
+ * tl_cg_loop, bit for bit.  Limits: n_last <= 16384 (TL_ERR_ARG otherwise), a
+ * chunk with all four faces external, kernel timing off for the call.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device race.  tl_loop_errors does not see its histories. */
+int tl_cg_loop_noalloc(tl_loop_state *st, double *cg_alphas, double *cg_betas);
+
 /* PPCG outer iterations after the switch (tea_solve.f90:289-331): calc_w,
  * store_r, zrnorm, calc_ur, `inner` sweeps with coefficients alphas/betas
  * (length inner), calc_rrn, calc_p.  inner_iters is incremented by `inner`

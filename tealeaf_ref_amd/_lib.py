@@ -79,7 +79,7 @@ def lib():
             "tl_ppcg_init": [I, PD], "tl_ppcg_init_sd": [D], "tl_ppcg_inner": [PD, PD, I, I],
             "tl_ppcg_store_r": [], "tl_ppcg_calc_zrnorm": [PD], "tl_ppcg_calc_rrn": [PD],
             "tl_ppcg_update_z": [], "tl_jacobi_solve": [PD],
-            "tl_cg_loop": [P(LoopState), PD, PD],
+            "tl_cg_loop": [P(LoopState), PD, PD], "tl_cg_loop_noalloc": [P(LoopState), PD, PD],
             "tl_ppcg_loop": [P(LoopState), PD, PD, I, D, P(I)],
             "tl_jacobi_loop": [P(LoopState)], "tl_loop_errors": [I, I, PD],
             "tl_cheby_run": [PD, PD, I, I, I, PD],

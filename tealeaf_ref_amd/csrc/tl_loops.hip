@@ -723,6 +723,87 @@ extern "C" int tl_cg_loop(tl_loop_state *st, double *cg_alphas, double *cg_betas
   return run_cg_loop<double, TL_V64>(bb, st, cg_alphas, cg_betas);
 }
 
+// ---------------------------------------------------------------------------
+// tl_cg_loop_noalloc: the fused CG loop with its scalars (control block,
+// per-iteration histories, partial-sum slots) in a file-scope device array
+// instead of library allocations, for callers that cannot allocate.  Each
+// device holds its own copy of the array until the process ends; every call
+// on a device shares it, so calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_MAX_ITERS = 16384;              // largest n_last accepted
+constexpr long NOALLOC_HIST = NOALLOC_MAX_ITERS + 4;  // entries per history
+constexpr size_t NOALLOC_CTL_BYTES = 256;
+// 256 B + (4 x 16388 + 6 x 2048) doubles = 622,976 bytes per device
+constexpr size_t NOALLOC_BYTES =
+    NOALLOC_CTL_BYTES + sizeof(double) * (4 * NOALLOC_HIST + (size_t)NPART * MAX_GRID);
+static_assert(sizeof(Ctl) <= NOALLOC_CTL_BYTES, "control block size");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+// Device address of `scratch` per device (the host-side name is not one).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void *scratch_addr[NOALLOC_MAX_DEVICES] = {nullptr};
+
+int scratch_base(int device, void **out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    return fail(TL_ERR_ARG, "tl_cg_loop_noalloc: device index above 63");
+  if (!scratch_addr[device]) {
+    // the symbol's address on `device`: make it current for the query
+    int cur = 0;
+    TL_HIP(hipGetDevice(&cur));
+    if (cur != device) TL_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) TL_HIP(hipSetDevice(cur));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(TL_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    scratch_addr[device] = p;
+  }
+  *out = scratch_addr[device];
+  return TL_OK;
+}
+
+__global__ void k_zero_bytes(unsigned char *__restrict__ p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+}  // namespace
+
+extern "C" int tl_cg_loop_noalloc(tl_loop_state *st, double *cg_alphas, double *cg_betas) {
+  TL_REQUIRE_INIT();
+  int rc = check_state(st);
+  if (rc) return rc;
+  if (st->n_last > NOALLOC_MAX_ITERS)
+    return fail(TL_ERR_ARG, "tl_cg_loop_noalloc: n_last above " +
+                                std::to_string(NOALLOC_MAX_ITERS) + " (its history size)");
+  State &s = S();
+  if (!s.zb_all)
+    return fail(TL_ERR_ARG,
+                "tl_cg_loop_noalloc: needs a chunk with all four faces external (fused path)");
+  void *base = nullptr;
+  if ((rc = scratch_base(s.device, &base))) return rc;
+  LoopMem m;
+  m.ctl = (Ctl *)base;
+  m.hrro = (double *)((unsigned char *)base + NOALLOC_CTL_BYTES);
+  m.halpha = m.hrro + NOALLOC_HIST;
+  m.hbeta = m.halpha + NOALLOC_HIST;
+  m.herr = m.hbeta + NOALLOC_HIST;
+  m.parts = m.herr + NOALLOC_HIST;
+  // the control block starts cleared (a kernel, not a memset node)
+  hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(64), 0, s.stream, (unsigned char *)m.ctl,
+                     (int)NOALLOC_CTL_BYTES);
+  TL_HIP(hipGetLastError());
+  // kernel timing creates events on demand: off for this call
+  const bool prof = s.prof;
+  s.prof = false;
+  LoopBufs<double> bb = fp64_bufs();
+  rc = run_cg_loop<double, TL_V64>(bb, st, cg_alphas, cg_betas, nullptr, &m);
+  s.prof = prof;
+  return rc;
+}
+
 extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const double *betas,
                             int inner, double theta, int *inner_iters) {
   TL_REQUIRE_INIT();
