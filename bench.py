@@ -77,13 +77,12 @@ def model_bytes_jacobi(rows):
                for r in rows)
 
 
-def measured_traffic(kernel, cells, profile="cg_fp64_4096_hbm_traffic.md"):
-    """HBM bytes per launch of a kernel at 4096^2 from a committed counter
-    summary (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated), or
-    None."""
-    if cells != 4096:
-        return None
-    path = os.path.join(ROOT, "profiles", profile)
+def measured_traffic(kernel, cells, profile="cg_fp64_{n}_hbm_traffic.md"):
+    """HBM bytes per launch of a kernel on a cells x cells mesh from the
+    committed counter summary `profile` ({n} stands for cells; rocprofv3
+    FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated), or None where no
+    summary covers that size."""
+    path = os.path.join(ROOT, "profiles", profile.format(n=cells))
     try:
         text = open(path).read()
     except OSError:
@@ -144,19 +143,46 @@ def gpu_run(cells, solver, steps, warmup, workdir, **deck_kw):
 # tl_profile_get; a PPCG sweep pass, Chebyshev pass or Jacobi pass does two
 # sweeps or iterations for the bytes of one (plus, for Jacobi, the store of
 # the first iteration's u)
+# the PPCG sweep class: one first pass (48 B: r, Kx, Ky in; sd, rtemp, utemp
+# out), three middle passes (64 B) and one last pass (64 B: z the only
+# output) per outer iteration of 10 inner steps
+PPCG_PASS_BYTES_PER_CELL = (48.0 + 3 * 64.0 + 64.0) / 5
 CLASS_BYTES = {0: ("cg calc_w", 48.0), 1: ("cg calc_ur", 48.0),
-               2: ("ppcg inner sweep pass", 64.0), 3: ("chebyshev pass", 56.0),
+               2: ("ppcg inner sweep pass", PPCG_PASS_BYTES_PER_CELL), 3: ("chebyshev pass", 56.0),
                4: ("jacobi pass", 48.0), 5: ("ppcg calc_w", 48.0),
                6: ("ppcg calc_ur + store_r", 56.0)}
 # the kernel of each class and the committed FETCH_SIZE / WRITE_SIZE summary
-# that holds its measured HBM bytes per launch at 4096^2
-CLASS_TRAFFIC = {0: ("k_loop_w", "cg_fp64_4096_hbm_traffic.md"),
-                 1: ("k_cg_ur", "cg_fp64_4096_hbm_traffic.md"),
-                 2: ("k_pp_sweep2", "ppcg_fp64_4096_hbm_traffic.md"),
-                 3: ("k_cheby_pair", "cheby_fp64_4096_hbm_traffic.md"),
-                 4: ("k_jl_pair", "jacobi_fp64_4096_kernel_trace.md"),
-                 5: ("k_loop_w", "ppcg_fp64_4096_hbm_traffic.md"),
-                 6: ("k_pp_ur", "ppcg_fp64_4096_hbm_traffic.md")}
+# that holds its measured HBM bytes per launch ({n}: the mesh size)
+CLASS_TRAFFIC = {0: ("k_loop_w", "cg_fp64_{n}_hbm_traffic.md"),
+                 1: ("k_cg_ur", "cg_fp64_{n}_hbm_traffic.md"),
+                 2: ("k_pp_sweep2", "ppcg_fp64_{n}_hbm_traffic.md"),
+                 3: ("k_cheby_pair", "cheby_fp64_{n}_hbm_traffic.md"),
+                 4: ("k_jl_pair", "jacobi_fp64_{n}_hbm_traffic.md"),
+                 5: ("k_loop_w", "ppcg_fp64_{n}_hbm_traffic.md"),
+                 6: ("k_pp_ur", "ppcg_fp64_{n}_hbm_traffic.md")}
+
+
+def class_rates(prof, cells, classes=CLASS_BYTES, traffic=CLASS_TRAFFIC):
+    """Per-class launch time and HBM rates from HIP-event samples `prof`
+    (class -> (timed launches, ms, idle launches)): algorithmic bytes, and
+    the counter-measured bytes where a committed summary covers this size."""
+    out = {}
+    for cls, (name, bpc) in classes.items():
+        n, ms, idle = prof.get(cls, (0, 0.0, 0))
+        if not n:
+            continue
+        k = {"timed_launches": n, "idle_launches": idle, "avg_launch_ms": ms / n,
+             "achieved_gbs": bpc * cells * cells / (ms / n * 1e-3) / 1e9, "bytes_per_cell": bpc}
+        k["frac"] = k["achieved_gbs"] / PEAK_HBM_GBS
+        kname, prof_file = traffic[cls]
+        t = measured_traffic(kname, cells, prof_file)
+        if t:
+            k["measured_bytes_per_launch"] = t
+            k["measured_gbs"] = t / (ms / n * 1e-3) / 1e9
+            k["measured_frac"] = k["measured_gbs"] / PEAK_HBM_GBS
+            k["traffic_source"] = "profiles/" + prof_file.format(n=cells)
+        out[name] = k
+    return out
 
 
 def work_launches(solver, rows, inner=0, max_iters=10000):
@@ -205,22 +231,7 @@ def solver_runs(cells, steps, warmup):
         rec["model_bytes_per_cell_step"] = model / steps
         rec["model_gbs"] = model * cells * cells / elapsed / 1e9
         rec["model_frac"] = rec["model_gbs"] / PEAK_HBM_GBS
-        kern = {}
-        for cls, (name, bpc) in CLASS_BYTES.items():
-            n, ms, idle = prof[cls]
-            if n:
-                k = {"timed_launches": n, "idle_launches": idle, "avg_launch_ms": ms / n,
-                     "achieved_gbs": bpc * cells * cells * n / (ms * 1e-3) / 1e9,
-                     "bytes_per_cell": bpc}
-                kname, prof_file = CLASS_TRAFFIC[cls]
-                traffic = measured_traffic(kname, cells, prof_file)
-                if traffic:
-                    # the same launches in the HBM bytes the counters measured
-                    k["measured_bytes_per_launch"] = traffic
-                    k["measured_gbs"] = traffic / (ms / n * 1e-3) / 1e9
-                    k["measured_frac"] = k["measured_gbs"] / PEAK_HBM_GBS
-                    k["traffic_source"] = "profiles/" + prof_file
-                kern[name] = k
+        kern = class_rates(prof, cells)
         rec["kernels"] = kern
         # the step's rate in counter-measured bytes: launches that did work x
         # the measured bytes per launch of their class, over the step time;
@@ -329,34 +340,25 @@ def run_deck_timed(deck, steps, warmup, workdir, want_u=False, prof_out=None, re
     return elapsed, rows, u
 
 
-# kernel classes of the FP32 inner PPCG of a mixed-precision solve: name,
-# algorithmic bytes per cell (FP32 fields) and the kernel's line in
-# profiles/ppcg_mixed_4096_hbm_traffic.md (the FP64 refinements, a few per
-# step, are too rare for sampled events)
-MIXED_PPCG_CLASSES = {2: ("fp32 ppcg inner sweep pass", 32.0, "k_pp_sweep2<float>"),
-                      5: ("fp32 ppcg calc_w", 24.0, "k_loop_w<float>"),
-                      6: ("fp32 ppcg calc_ur + store_r", 28.0, "k_pp_ur<float>")}
+# kernel classes of the FP32 inner PPCG of a mixed-precision solve: name and
+# algorithmic bytes per cell (FP32 fields, half the FP64 figures), and the
+# kernel's line in profiles/ppcg_mixed_{n}_hbm_traffic.md (the FP64
+# refinements, a few per step, are too rare for sampled events)
+MIXED_PPCG_BYTES = {2: ("fp32 ppcg inner sweep pass", PPCG_PASS_BYTES_PER_CELL / 2),
+                    5: ("fp32 ppcg calc_w", 24.0), 6: ("fp32 ppcg calc_ur + store_r", 28.0)}
+MIXED_PPCG_TRAFFIC = {2: ("k_pp_sweep2<float>", "ppcg_mixed_{n}_hbm_traffic.md"),
+                      5: ("k_loop_w<float>", "ppcg_mixed_{n}_hbm_traffic.md"),
+                      6: ("k_pp_ur<float>", "ppcg_mixed_{n}_hbm_traffic.md")}
+
+
+MIXED_CG_BYTES = {0: ("fp32 cg calc_w", 24.0), 1: ("fp32 cg calc_ur", 24.0)}
+MIXED_CG_TRAFFIC = {0: ("k_loop_w<float>", "ppcg_mixed_{n}_hbm_traffic.md"),
+                    1: ("k_cg_ur<float>", "ppcg_mixed_{n}_hbm_traffic.md")}
 
 
 def mixed_kernel_rates(prof, cells):
-    """Per-class launch time and HBM rates of a mixed-precision PPCG run:
-    algorithmic bytes, and the counter-measured bytes where a summary holds
-    them (4096^2)."""
-    out = {}
-    for cls, (name, bpc, kname) in MIXED_PPCG_CLASSES.items():
-        n, ms, idle = prof.get(cls, (0, 0.0, 0))
-        if not n:
-            continue
-        k = {"timed_launches": n, "idle_launches": idle, "avg_launch_ms": ms / n,
-             "achieved_gbs": bpc * cells * cells / (ms / n * 1e-3) / 1e9, "bytes_per_cell": bpc}
-        traffic = measured_traffic(kname, cells, "ppcg_mixed_4096_hbm_traffic.md")
-        if traffic:
-            k["measured_bytes_per_launch"] = traffic
-            k["measured_gbs"] = traffic / (ms / n * 1e-3) / 1e9
-            k["measured_frac"] = k["measured_gbs"] / PEAK_HBM_GBS
-            k["traffic_source"] = "profiles/ppcg_mixed_4096_hbm_traffic.md"
-        out[name] = k
-    return out
+    """Per-class launch time and HBM rates of a mixed-precision PPCG run."""
+    return class_rates(prof, cells, MIXED_PPCG_BYTES, MIXED_PPCG_TRAFFIC)
 
 
 def host_true_residual(cells, rx, ry):
@@ -438,7 +440,7 @@ def precision_comparison(cells, inner_steps=10, target=1e-10, later_step=False):
         with tempfile.TemporaryDirectory() as wd:
             run_deck_timed(deck_of(name, solver, extra, 1, cells=256), 1, 0, wd)
         log(f"[bench] {cells}^2 {name}: start")
-        prof = {} if name == "mixed_ppcg" else None
+        prof = {}
         with tempfile.TemporaryDirectory() as wd:
             elapsed, rows, u, tres = run_deck_timed(deck_of(name, solver, extra, 1), 1, 0, wd,
                                                     want_u=True, prof_out=prof,
@@ -458,8 +460,12 @@ def precision_comparison(cells, inner_steps=10, target=1e-10, later_step=False):
             # HBM rate in the algorithmic bytes of the reference's dataflow
             rec["model_gbs"] = (model_bytes_per_cell(solver, rows) * cells * cells / elapsed
                                 / 1e9)
-        if prof:
+        if name == "mixed_ppcg":
             rec["kernels"] = mixed_kernel_rates(prof, cells)
+        elif name == "mixed_cg":
+            rec["kernels"] = class_rates(prof, cells, MIXED_CG_BYTES, MIXED_CG_TRAFFIC)
+        elif not extra:
+            rec["kernels"] = class_rates(prof, cells)
         summ = [last["volume"], last["mass"], last["energy"], last["temperature"]]
         if ref_u is None:
             ref_u, ref_sum = u, summ
@@ -742,16 +748,24 @@ def cpu_baseline(cells, solver, gpu_iters_per_step, cap, cap_one_thread):
 
 def profile_run(cells, iters, solver="cg", copies=4):
     """Short run for hardware-counter profiles: one timestep of the seeded
-    deck (CG, or PPCG with 10 inner steps) capped at `iters` iterations, then
-    `copies` launches of the plain copy kernel (tl_ppcg_store_r: r_store = r
-    over the interior, 8 B read + 8 B written per cell), whose counters
-    calibrate FETCH_SIZE / WRITE_SIZE."""
+    deck capped at `iters` iterations, then `copies` launches of the plain
+    copy kernel (tl_ppcg_store_r: r_store = r over the interior, 8 B read +
+    8 B written per cell), whose counters calibrate FETCH_SIZE / WRITE_SIZE.
+    Solvers: "cg"; "ppcg" (10 inner steps; the switch to PPCG at the 100th
+    CG iteration: tl_ch_cg_presteps 100 with an epslim no error exceeds);
+    "mixed_ppcg" (FP32 PPCG under FP64 refinement, inverse-density
+    coefficient, at most 2 refinements of at most `iters` iterations)."""
     from tealeaf_ref_amd import _lib
     from tealeaf_ref_amd.deckgen import bench_deck
     from tealeaf_ref_amd.host import TeaHost
-    kw = {"inner_steps": 10} if solver == "ppcg" else {}
-    deck = bench_deck(cells, solver=solver, eps=1e-15, max_iters=iters, end_step=1,
-                      summary_frequency=1, **kw)
+    kw = {}
+    if solver == "ppcg":
+        kw = {"inner_steps": 10, "presteps": 100, "epslim": 1e30}
+    elif solver == "mixed_ppcg":
+        kw = {"inner_steps": 10, "coefficient": "inverse_density",
+              "extra": ("tl_mixed_precision", "tl_mixed_max_refinements=2")}
+    deck = bench_deck(cells, solver="ppcg" if solver == "mixed_ppcg" else solver, eps=1e-15,
+                      max_iters=iters, end_step=1, summary_frequency=1, **kw)
     with tempfile.TemporaryDirectory() as wd:
         path = os.path.join(wd, "tea.in")
         with open(path, "w") as f:
@@ -782,7 +796,7 @@ def main():
     ap.add_argument("--mixed-sizes", default="4096,8192",
                     help="comma-separated mesh sizes of the FP64 vs FP32-refined comparison "
                          "('' to skip)")
-    ap.add_argument("--profile-solver", default="cg", choices=["cg", "ppcg"],
+    ap.add_argument("--profile-solver", default="cg", choices=["cg", "ppcg", "mixed_ppcg"],
                     help="solver of the counter-profiling mode")
     ap.add_argument("--profile-iters", type=int, default=0,
                     help="counter-profiling mode: one CG step capped at this many iterations "

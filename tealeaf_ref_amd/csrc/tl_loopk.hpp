@@ -311,6 +311,14 @@ __global__ __launch_bounds__(BLOCK, TL_W_WAVES) void k_loop_w(Geo G, int n, int 
   if (threadIdx.x == 0) out_part[blockIdx.x] = acc;
 }
 
+// Rows per group of the unpreconditioned CG update pass (k_cg_ur): 8 rows
+// of loads in flight per lane made the pass 3 % faster than one row at
+// 4096^2 and 8192^2.
+#ifndef TL_UR_ROWS
+#define TL_UR_ROWS 8
+#endif
+constexpr int UR_ROWS = TL_UR_ROWS;
+
 // CG iteration n, second pass: alpha = rro/pw; u += alpha p; r -= alpha w;
 // (z = Mi r); partials of r.r or r.z (kernels/tea_leaf_cg_kernel.f90:237-300)
 template <bool NT, typename T, int V>
@@ -336,7 +344,37 @@ __global__ __launch_bounds__(BLOCK) void k_cg_ur(Geo G, int n, T *__restrict__ u
   for (long t = it.t; t < it.t_end; ++t) {
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
     if (tt.c >= G.nx) continue;
-    for (int k = tt.y0; k < tt.y1; ++k) {
+    int k = tt.y0;
+    if (!diag) {
+      // groups of UR_ROWS rows with every load issued before the first
+      // store, so that each lane has 4 x UR_ROWS loads in flight; the r.r
+      // partial still adds the rows in walking order
+      for (; k + UR_ROWS <= tt.y1; k += UR_ROWS) {
+        VT uu[UR_ROWS], pp[UR_ROWS], rr[UR_ROWS], ww[UR_ROWS];
+#pragma unroll
+        for (int q = 0; q < UR_ROWS; ++q) {
+          const int y = rev ? tt.y1 - 1 - (k + q - tt.y0) : k + q;
+          const long i = (long)y * P + tt.c;
+          uu[q] = dev::ldv<NT, T, V>(u + i);
+          pp[q] = dev::ldv<(NT && TL_LD_NT_UR), T, V>(p + i);
+          rr[q] = dev::ldv<NT, T, V>(r + i);
+          ww[q] = dev::ldv<(NT && TL_LD_NT_UR), T, V>(w + i);
+        }
+#pragma unroll
+        for (int q = 0; q < UR_ROWS; ++q) {
+          const int y = rev ? tt.y1 - 1 - (k + q - tt.y0) : k + q;
+          const VT uv = uu[q] + a * pp[q];
+          const VT rv = rr[q] - a * ww[q];
+          dev::stv<(NT && (TL_ST_NT_LOOP || (TL_ST_NT_U && sizeof(T) == 8))), T, V>(
+              u + (long)y * P, tt.c, G.nx, uv);
+          dev::stv<(NT && TL_ST_NT_LOOP), T, V>(r + (long)y * P, tt.c, G.nx, rv);
+#pragma unroll
+          for (int e = 0; e < V; ++e)
+            if (tt.c + e < G.nx) acc += (double)rv[e] * (double)rv[e];
+        }
+      }
+    }
+    for (; k < tt.y1; ++k) {
       const int y = rev ? tt.y1 - 1 - (k - tt.y0) : k;
       const long i = (long)y * P + tt.c;
       const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<(NT && TL_LD_NT_UR), T, V>(p + i);
@@ -554,12 +592,19 @@ __global__ __launch_bounds__(BLOCK) void k_pp_sweep(
 // Inner sweeps i and i+1 in one pass (tl_sweep2.hpp): reads sdin, rtin
 // (r on the first sweep), utemp; writes sdout, rtout, utemp.  FIRST / LAST
 // as first_sweep / last_sweep in k_pp_sweep, applying to sweep i / sweep
-// i+1; DIAG: the Jacobi-diagonal preconditioner.  They are template
+// i+1, except that the LAST pass writes z alone: it runs only inside the
+// fused solves, where sd, rtemp and utemp are dead after the sweeps (unlike
+// the per-routine tl_ppcg_inner, whose caller reads utemp in update_z).
+// DIAG: the Jacobi-diagonal preconditioner.  They are template
 // parameters, so that each of the eight kernels holds only the pointers and
 // registers it uses (the middle passes of an outer iteration need neither r,
 // r_store, z nor Mi).
 #ifndef TL_PF_PAIR
 #define TL_PF_PAIR 0
+#endif
+
+#ifndef TL_SWEEP_FIRST_PLAIN
+#define TL_SWEEP_FIRST_PLAIN 1
 #endif
 
 template <typename VT>
@@ -585,9 +630,15 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_W
   dev::TileIter it(G.tiles, G.blocks);
   for (long t = it.t; t < it.t_end; ++t) {
     const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    // first pass without a preconditioner: src is r, read here one row
+    // ahead of rtA's read of the same row.  In FP32 plain loads let the
+    // second come from the L2 (mixed PPCG step 1.4 % faster at 4096^2); in
+    // FP64 the streaming loads measured faster (1.9 %)
+    constexpr bool NTF =
+        NT && !(FIRST && !DIAG && sizeof(T) == 4 && TL_SWEEP_FIRST_PLAIN);
     auto opA = [&](int col, int row) -> VT {
       const long i = (long)row * P + col;
-      return FIRST ? dev::ldv<NT, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
+      return FIRST ? dev::ldv<NTF, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
     };
     auto opA1 = [&](int col, int row) -> T {
       const long i = (long)row * P + col;
@@ -595,7 +646,7 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_W
     };
     auto rtA = [&](int col, int row) -> VT {
       const long i = (long)row * P + col;
-      return FIRST ? dev::ldv<NT, T, V>(r + i) : dev::ldv<NT, T, V>(rtin + i);
+      return FIRST ? dev::ldv<NTF, T, V>(r + i) : dev::ldv<NT, T, V>(rtin + i);
     };
     auto rtA1 = [&](int col, int row) -> T {
       const long i = (long)row * P + col;
@@ -619,11 +670,15 @@ __global__ __launch_bounds__(BLOCK, sizeof(T) == 4 ? TL_PAIR_WAVES32 : TL_PAIR_W
           const VT s2 = DIAG ? a2 * s1 + (b2 * mi) * r2 : a2 * s1 + b2 * r2;
           const VT ut0 = FIRST ? s0 : x.ut;
           const VT ut = (ut0 + s1) + s2;
-          dev::stv<(NT && ST), T, V>(rtout + row, tt.c, G.nx, r2);
-          dev::stv<(NT && ST), T, V>(sdout + row, tt.c, G.nx, s2);
-          dev::stv<(NT && (ST || (TL_ST_NT_UTEMP && sizeof(T) == 8))), T, V>(utemp + row, tt.c,
-                                                                              G.nx, ut);
-          if (LAST) {
+          if (!LAST) {
+            dev::stv<(NT && ST), T, V>(rtout + row, tt.c, G.nx, r2);
+            dev::stv<(NT && ST), T, V>(sdout + row, tt.c, G.nx, s2);
+            dev::stv<(NT && (ST || (TL_ST_NT_UTEMP && sizeof(T) == 8))), T, V>(utemp + row, tt.c,
+                                                                                G.nx, ut);
+          } else {
+            // the last pass of a solve's sweeps: only z = utemp (update_z)
+            // is read afterwards; the next first pass starts sd, rtemp and
+            // utemp afresh from r
             dev::stv<(NT && ST), T, V>(z + row, tt.c, G.nx, ut);
 #pragma unroll
             for (int e = 0; e < V; ++e)

@@ -15,27 +15,46 @@ import bench  # noqa: E402
 CELLS = 4096 * 4096
 
 
-def test_every_kernel_class_has_measured_traffic():
+@pytest.mark.parametrize("n", [4096, 8192])
+def test_every_kernel_class_has_measured_traffic(n):
+    cells = n * n
     for cls, (kernel, profile) in bench.CLASS_TRAFFIC.items():
-        t = bench.measured_traffic(kernel, 4096, profile)
+        t = bench.measured_traffic(kernel, n, profile)
+        if n == 8192 and cls in (3, 4):   # Chebyshev and Jacobi: profiled at 4096^2 only
+            assert t is None
+            continue
         assert t, (cls, kernel, profile)
         name, bpc = bench.CLASS_BYTES[cls]
         # the counters see at least the algorithmic bytes, and ghost rows,
         # columns and the extra loads of a batch's last pass add under 20 %
-        assert bpc * CELLS * 0.99 <= t <= bpc * CELLS * 1.2, (name, t / CELLS, bpc)
+        assert bpc * cells * 0.99 <= t <= bpc * cells * 1.2, (name, t / cells, bpc)
 
 
-@pytest.mark.parametrize("cls", sorted(bench.MIXED_PPCG_CLASSES))
-def test_mixed_classes_have_measured_traffic(cls):
-    name, bpc, kernel = bench.MIXED_PPCG_CLASSES[cls]
-    t = bench.measured_traffic(kernel, 4096, "ppcg_mixed_4096_hbm_traffic.md")
-    assert t and bpc * CELLS * 0.99 <= t <= bpc * CELLS * 1.2, (name, t)
+@pytest.mark.parametrize("n", [4096, 8192])
+@pytest.mark.parametrize("table", ["ppcg", "cg"])
+def test_mixed_classes_have_measured_traffic(n, table):
+    bytes_, traffic = ((bench.MIXED_PPCG_BYTES, bench.MIXED_PPCG_TRAFFIC) if table == "ppcg"
+                       else (bench.MIXED_CG_BYTES, bench.MIXED_CG_TRAFFIC))
+    for cls, (name, bpc) in bytes_.items():
+        kernel, profile = traffic[cls]
+        t = bench.measured_traffic(kernel, n, profile)
+        assert t and bpc * n * n * 0.99 <= t <= bpc * n * n * 1.2, (name, t)
 
 
 def test_roofline_traffic_of_the_dominant_kernel():
     t = bench.measured_traffic("k_loop_w", 4096)
     assert t and abs(t / CELLS - bench.CG_W_BYTES_PER_CELL) < 3.0
-    assert bench.measured_traffic("k_loop_w", 2048) is None  # only the profiled size
+    assert bench.measured_traffic("k_loop_w", 2048) is None  # no summary at that size
+
+
+def test_class_rates_from_event_samples():
+    """Rates of a class from (timed launches, ms, idle launches): algorithmic
+    bytes over the mean launch, and the measured bytes where profiled."""
+    r = bench.class_rates({0: (10, 1.3, 1)}, 4096)["cg calc_w"]
+    assert abs(r["avg_launch_ms"] - 0.13) < 1e-12
+    assert abs(r["achieved_gbs"] - 48.0 * CELLS / 0.13e-3 / 1e9) < 1e-6
+    assert r["measured_gbs"] == pytest.approx(bench.measured_traffic("k_loop_w", 4096) / 0.13e-3 / 1e9)
+    assert "measured_gbs" not in bench.class_rates({0: (10, 1.3, 1)}, 2048)["cg calc_w"]
 
 
 def test_more_than_one_gpu_is_refused():
