@@ -90,6 +90,16 @@ int tl_set_ppcg_active(int active);
 int tl_upload(int field_id, const double *host, int host_halo);
 int tl_download(int field_id, double *host, int host_halo);
 
+/* The interior ny x nx cells of a field to/from an array in device memory
+ * of the library's GPU, row stride ld doubles (ld >= nx): a device-to-device
+ * copy, done when the call returns.  No counterpart in the reference: for
+ * callers whose data already lives on the GPU (a torch tensor), so that it
+ * does not travel through the host.  The caller finishes any work writing
+ * the array (e.g. synchronizes its stream) first.  A pointer that is not
+ * device memory of this GPU is refused with TL_ERR_ARG. */
+int tl_upload_device(int field_id, const double *dev, long ld);
+int tl_download_device(int field_id, double *dev, long ld);
+
 /* Block until all queued work is done. */
 int tl_synchronize(void);
 
@@ -321,6 +331,9 @@ int tl_snapshot_fill(int slot, double value);
 /* Host <-> slot copies, as tl_upload / tl_download. */
 int tl_snapshot_upload(int slot, const double *host, int host_halo);
 int tl_snapshot_download(int slot, double *host, int host_halo);
+/* Device array <-> slot copies of the interior cells, as tl_upload_device. */
+int tl_snapshot_upload_device(int slot, const double *dev, long ld);
+int tl_snapshot_download_device(int slot, double *dev, long ld);
 /* grad += dJ/dlog(rho) of one step: for every cell c and each of its internal
  * faces f (neighbour b), -r_f (lambda_c - lambda_b)(u_c - u_b) dK_f/dlog(rho_c)
  * with r_f = rx or ry of the last tl_common_init and dK_f/dlog(rho_c) =

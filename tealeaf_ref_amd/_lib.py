@@ -91,6 +91,11 @@ def lib():
             "tl_device_bytes": [PD], "tl_profile_enable": [I], "tl_profile_get": [I, P(C.c_long), PD],
             "tl_profile_idle": [I, P(C.c_long)],
             "tl_profile_reset": [], "tl_hbm_stream": [C.c_long, I, PD, PD],
+            # device addresses (e.g. torch's data_ptr()) and a row stride
+            "tl_upload_device": [I, C.c_void_p, C.c_long],
+            "tl_download_device": [I, C.c_void_p, C.c_long],
+            "tl_snapshot_upload_device": [I, C.c_void_p, C.c_long],
+            "tl_snapshot_download_device": [I, C.c_void_p, C.c_long],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

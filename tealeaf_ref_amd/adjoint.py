@@ -77,6 +77,7 @@ class TeaLeafSteps:
         self.reflective = bool(reflective)
         self.iterations = 0            # solver iterations of the last forward/backward
         self.progress = None           # optional callable(str), told of every step
+        self.device = int(device)
         self.dev = L.Device(self.nx, self.ny, 1, device)
         self.dev.set_options(coefficient, L.PREC_NONE, self.reflective)
         self.dev.upload("energy1", _pad(e0))
@@ -189,6 +190,32 @@ class TeaLeafSteps:
         if th.shape != (self.ny, self.nx):
             raise ValueError(f"theta must have shape {(self.ny, self.nx)}")
         self.dev.upload("density", _pad(np.exp(th)))
+        self._run_forward()
+        return self.dev.download("u")[1:-1, 1:-1].copy()
+
+    def forward_device(self, theta):
+        """forward() on a torch tensor theta (ny, nx) on this chunk's GPU: exp
+        on the GPU, density and u copied device to device; returns u as a new
+        float64 tensor on the same GPU."""
+        import torch
+        th = self._device_tensor(theta, "theta")
+        rho = torch.exp(th).contiguous()
+        torch.cuda.current_stream(th.device).synchronize()
+        L.call("tl_upload_device", L.FIELD["density"], rho.data_ptr(), self.nx)
+        self._run_forward()
+        u = torch.empty((self.ny, self.nx), dtype=torch.float64, device=th.device)
+        L.call("tl_download_device", L.FIELD["u"], u.data_ptr(), self.nx)
+        return u
+
+    def _device_tensor(self, t, what):
+        import torch
+        if t.shape != (self.ny, self.nx):
+            raise ValueError(f"{what} must have shape {(self.ny, self.nx)}")
+        if not t.is_cuda or t.device.index != self.device:
+            raise ValueError(f"{what} must be on cuda:{self.device}")
+        return t.detach().to(torch.float64).contiguous()
+
+    def _run_forward(self):
         L.call("tl_snapshot_load", self.E0, L.FIELD["energy1"])
         self.iterations = 0
         self.sweeps = 0
@@ -200,7 +227,6 @@ class TeaLeafSteps:
             if self.progress:
                 self.progress(f"forward step {n}/{self.steps}, {self.iterations} iterations")
         self._forward_done = True
-        return self.dev.download("u")[1:-1, 1:-1].copy()
 
     def backward(self, grad_u):
         """dJ/dtheta from dJ/du_N (ny, nx) for the theta of the last forward."""
@@ -211,7 +237,30 @@ class TeaLeafSteps:
             raise ValueError(f"grad_u must have shape {(self.ny, self.nx)}")
         host = np.zeros((self.ny + 2, self.nx + 2))
         host[1:-1, 1:-1] = g
+        # the slot's halo stays zero: g_N lives on the interior
+        L.call("tl_snapshot_fill", self.RHS, 0.0)
         L.call("tl_snapshot_upload", self.RHS, host.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        self._run_backward()
+        out = np.zeros((self.ny + 2, self.nx + 2))
+        L.call("tl_snapshot_download", self.GRAD, out.ctypes.data_as(C.POINTER(C.c_double)), 1)
+        return out[1:-1, 1:-1].copy()
+
+    def backward_device(self, grad_u):
+        """backward() on a torch tensor dJ/du_N on this chunk's GPU, copied
+        device to device; returns dJ/dtheta as a new float64 tensor there."""
+        import torch
+        if not self._forward_done:
+            raise RuntimeError("backward needs a forward of the same TeaLeafSteps first")
+        g = self._device_tensor(grad_u, "grad_u")
+        torch.cuda.current_stream(g.device).synchronize()
+        L.call("tl_snapshot_fill", self.RHS, 0.0)
+        L.call("tl_snapshot_upload_device", self.RHS, g.data_ptr(), self.nx)
+        self._run_backward()
+        out = torch.empty((self.ny, self.nx), dtype=torch.float64, device=g.device)
+        L.call("tl_snapshot_download_device", self.GRAD, out.data_ptr(), self.nx)
+        return out
+
+    def _run_backward(self):
         L.call("tl_snapshot_fill", self.GRAD, 0.0)
         self.iterations = 0
         self.sweeps = 0
@@ -234,9 +283,6 @@ class TeaLeafSteps:
                 L.call("tl_snapshot_save", self.RHS, L.FIELD["u"])   # g_{n-1} = lambda_n
                 if self.progress:
                     self.progress(f"adjoint step {n}, {self.iterations} iterations")
-        out = np.zeros((self.ny + 2, self.nx + 2))
-        L.call("tl_snapshot_download", self.GRAD, out.ctypes.data_as(C.POINTER(C.c_double)), 1)
-        return out[1:-1, 1:-1].copy()
 
     # -- torch -------------------------------------------------------------------------
     def apply(self, theta):
@@ -291,13 +337,18 @@ def _function():
             @staticmethod
             def forward(ctx, theta, problem):
                 ctx.problem = problem
-                u = problem.forward(theta.detach().to("cpu", torch.float64).numpy())
-                return torch.from_numpy(u).to(device=theta.device, dtype=theta.dtype)
+                if theta.is_cuda:
+                    # stays on the GPU: device-to-device copies only
+                    return problem.forward_device(theta).to(theta.dtype)
+                u = problem.forward(theta.detach().to(torch.float64).numpy())
+                return torch.from_numpy(u).to(dtype=theta.dtype)
 
             @staticmethod
             def backward(ctx, grad_u):
-                g = ctx.problem.backward(grad_u.detach().to("cpu", torch.float64).numpy())
-                return torch.from_numpy(g).to(device=grad_u.device, dtype=grad_u.dtype), None
+                if grad_u.is_cuda:
+                    return ctx.problem.backward_device(grad_u).to(grad_u.dtype), None
+                g = ctx.problem.backward(grad_u.detach().to(torch.float64).numpy())
+                return torch.from_numpy(g).to(dtype=grad_u.dtype), None
 
         _FN = TeaLeafFunction.apply
     return _FN

@@ -157,6 +157,21 @@ extern "C" int tl_snapshot_download(int slot, double *host, int host_halo) {
   return copy_slot(slot, host, host_halo, false);
 }
 
+extern "C" int tl_snapshot_upload_device(int slot, const double *dev, long ld) {
+  TL_REQUIRE_INIT();
+  int rc = check_slot(slot);
+  if (rc) return rc;
+  return copy_interior_device(snap11(slot), const_cast<double *>(dev), ld, true,
+                              "tl_snapshot_upload_device");
+}
+
+extern "C" int tl_snapshot_download_device(int slot, double *dev, long ld) {
+  TL_REQUIRE_INIT();
+  int rc = check_slot(slot);
+  if (rc) return rc;
+  return copy_interior_device(snap11(slot), dev, ld, false, "tl_snapshot_download_device");
+}
+
 extern "C" int tl_adjoint_face_grad(int lambda_field, int u_slot, int e0_slot, int grad_slot) {
   TL_REQUIRE_INIT();
   State &s = S();

@@ -141,6 +141,11 @@ int finish_sums(const int *slots, int nslots, int n, double *out);
 // field of a TL_FIELD_* id (the current buffer of ping-ponged fields)
 Field *field_by_id(int id);
 
+// checks of a caller's ny x nx device array (row stride ld) and a
+// synchronous copy of the interior cells between it and p11 (tl_runtime.hip)
+int check_device_array(const double *p, long ld, const char *who);
+int copy_interior_device(double *p11, double *dev, long ld, bool up, const char *who);
+
 // Chebyshev iterations on the vector walk (tl_loops.hip): 1 = done,
 // 0 = not applicable here, < 0 = -(error code)
 int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_first, int count,

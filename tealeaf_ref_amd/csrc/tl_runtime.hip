@@ -333,6 +333,58 @@ extern "C" int tl_download(int field_id, double *host, int host_halo) {
   return copy_field(field_id, host, host_halo, false);
 }
 
+namespace tl {
+// An ny x nx array in device memory of the library's GPU with row stride ld
+// (in doubles): checked on the host, so that a host pointer or another GPU's
+// memory is refused instead of being handed to a copy.
+int check_device_array(const double *p, long ld, const char *who) {
+  State &s = S();
+  if (!p) return fail(TL_ERR_ARG, std::string(who) + ": null device pointer");
+  if (ld < s.nx) return fail(TL_ERR_ARG, std::string(who) + ": row stride below nx");
+  hipPointerAttribute_t a;
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(TL_ERR_ARG, std::string(who) + ": not device memory");
+  }
+  if (a.type != hipMemoryTypeDevice || a.device != s.device)
+    return fail(TL_ERR_ARG, std::string(who) + ": not device memory of the library's GPU");
+  return TL_OK;
+}
+
+// Interior cells of a device-side field base (pitched, halo offset) <->
+// a dense ny x nx device array with row stride ld; synchronous.
+int copy_interior_device(double *p11, double *dev, long ld, bool up, const char *who) {
+  State &s = S();
+  int rc = check_device_array(dev, ld, who);
+  if (rc) return rc;
+  if (up)
+    TL_HIP(hipMemcpy2DAsync(p11, s.pitch * sizeof(double), dev, ld * sizeof(double),
+                            (size_t)s.nx * sizeof(double), s.ny, hipMemcpyDeviceToDevice,
+                            s.stream));
+  else
+    TL_HIP(hipMemcpy2DAsync(dev, ld * sizeof(double), p11, s.pitch * sizeof(double),
+                            (size_t)s.nx * sizeof(double), s.ny, hipMemcpyDeviceToDevice,
+                            s.stream));
+  TL_HIP(hipStreamSynchronize(s.stream));
+  return TL_OK;
+}
+}  // namespace tl
+
+extern "C" int tl_upload_device(int field_id, const double *dev, long ld) {
+  TL_REQUIRE_INIT();
+  Field *f = field_by_id(field_id);
+  if (!f) return fail(TL_ERR_ARG, "unknown field id " + std::to_string(field_id));
+  return copy_interior_device(f->p11, const_cast<double *>(dev), ld, true, "tl_upload_device");
+}
+
+extern "C" int tl_download_device(int field_id, double *dev, long ld) {
+  TL_REQUIRE_INIT();
+  Field *f = field_by_id(field_id);
+  if (!f) return fail(TL_ERR_ARG, "unknown field id " + std::to_string(field_id));
+  return copy_interior_device(f->p11, dev, ld, false, "tl_download_device");
+}
+
 // --------------------------------------------------------------------------
 // C ABI: profiling
 // --------------------------------------------------------------------------

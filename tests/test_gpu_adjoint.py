@@ -102,20 +102,71 @@ def test_checkpointing_is_bit_identical(problems):
     assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
 
 
-def test_torch_function(problems):
+@pytest.mark.parametrize("dev", ["cpu", "cuda"])
+def test_torch_function(problems, dev):
+    """The autograd binding against the numpy calls.  A CPU tensor takes the
+    numpy path (identical bits); a GPU tensor stays on the GPU (exp on the
+    GPU, device-to-device copies), where exp may round a density one unit
+    differently from numpy's, hence a tolerance."""
     import torch
     theta, e0, target, win = case(13)
     p = problems(e0, 10.0, 10.0, 3, checkpoint_every=2)
-    dev = "cuda" if torch.cuda.is_available() else "cpu"
     th = torch.tensor(theta, device=dev, requires_grad=True)
     tg = torch.tensor(target, device=dev)
     u = p.apply(th)
+    assert u.device == th.device and u.dtype == th.dtype
     J = ((u[win] - tg[win]) ** 2).sum()
     J.backward()
+    assert th.grad.device == th.device
     u_np = p.forward(theta)
     g_np = p.backward(seed_grad(u_np, target, win))
-    assert np.array_equal(th.grad.cpu().numpy(), g_np)
-    assert abs(J.item() - objective(u_np, target, win)) <= 1e-12 * abs(J.item())
+    g = th.grad.cpu().numpy()
+    if dev == "cpu":
+        assert np.array_equal(g, g_np)
+        assert np.array_equal(u.detach().numpy(), u_np)
+    else:
+        assert np.allclose(u.detach().cpu().numpy(), u_np, rtol=1e-12, atol=0.0)
+        assert np.linalg.norm(g - g_np) <= 1e-9 * np.linalg.norm(g_np)
+    assert abs(J.item() - objective(u_np, target, win)) <= 1e-11 * abs(J.item())
+
+
+def test_device_copies():
+    """tl_upload_device / tl_download_device and the snapshot forms copy the
+    interior of a field bit for bit to and from torch tensors on the GPU
+    (row stride ld), and refuse a host pointer or a short row stride."""
+    import ctypes as C
+    import torch
+    nx, ny = 37, 29
+    d = L.Device(nx, ny, 2)
+    try:
+        rng = np.random.default_rng(3)
+        a = rng.standard_normal((ny, nx))
+        ta = torch.tensor(a, device="cuda")
+        L.call("tl_upload_device", L.FIELD["u0"], ta.data_ptr(), nx)
+        assert np.array_equal(d.download("u0")[2:-2, 2:-2], a)
+        # a strided source: every other column of a (ny, 2 nx) tensor
+        wide = torch.zeros((ny, 2 * nx), dtype=torch.float64, device="cuda")
+        wide[:, :nx] = ta * 2.0
+        L.call("tl_upload_device", L.FIELD["r"], wide.data_ptr(), 2 * nx)
+        assert np.array_equal(d.download("r")[2:-2, 2:-2], 2.0 * a)
+        out = torch.full((ny, nx), -1.0, dtype=torch.float64, device="cuda")
+        L.call("tl_download_device", L.FIELD["u0"], out.data_ptr(), nx)
+        assert np.array_equal(out.cpu().numpy(), a)
+        L.call("tl_snapshot_reserve", 1)
+        L.call("tl_snapshot_fill", 0, 0.0)
+        L.call("tl_snapshot_upload_device", 0, ta.data_ptr(), nx)
+        back = torch.zeros((ny, nx), dtype=torch.float64, device="cuda")
+        L.call("tl_snapshot_download_device", 0, back.data_ptr(), nx)
+        assert torch.equal(back, ta)
+        host = np.zeros((ny, nx))
+        with pytest.raises(L.TeaLeafError, match="not device memory"):
+            L.call("tl_upload_device", L.FIELD["u0"], host.ctypes.data, nx)
+        with pytest.raises(L.TeaLeafError, match="row stride"):
+            L.call("tl_download_device", L.FIELD["u0"], out.data_ptr(), nx - 1)
+        with pytest.raises(L.TeaLeafError, match="unknown field"):
+            L.call("tl_upload_device", 999, ta.data_ptr(), nx)
+    finally:
+        d.close()
 
 
 def test_mixed_precision_adjoint(problems):
