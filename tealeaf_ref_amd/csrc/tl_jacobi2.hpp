@@ -6,9 +6,11 @@
 // and the second needs u' at the neighbours of every cell, so a block
 // computes the first iteration ("stage A") one row ahead of the second
 // ("stage B"), on one ghost row beyond each end of its tile and on the ghost
-// column beside the block.  Ghost results are used and discarded.  For
-// chunks whose external faces have zero coefficients (the walk mirrors at
-// the mesh edge).
+// column beside the block.  Ghost results are used and discarded.  Outside
+// the mesh the walk mirrors (ZE = false: chunks whose external faces have
+// zero coefficients, where the value is multiplied by zero) or reads zero
+// (ZE = true: reflective faces, whose coefficients are not zero, as the
+// reference reads the halo of un, which it never writes).
 #pragma once
 
 #include "tl_sweep2.hpp"
@@ -30,7 +32,7 @@ __device__ inline double jacobi_cell(double s0, double l, double r, double d, do
 // the first iteration (u1) and after the second (u2).  opA(col, row) /
 // opA1(col, row) load u (V values / one value); src(col, row) / src1(col,
 // row) load u0.  The body must not load from memory written by this pass.
-template <bool NT, int V, typename OPA, typename OPA1, typename SRC, typename SRC1,
+template <bool NT, int V, bool ZE, typename OPA, typename OPA1, typename SRC, typename SRC1,
           typename BODY>
 __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
                                         const double *__restrict__ Kx,
@@ -67,14 +69,15 @@ __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
   auto face_b = [&](int y) -> long { return (long)(fwd ? y : y + 1) * P; };
   auto face_a = [&](int y) -> long { return (long)(fwd ? y + 1 : y) * P; };
 
+  const VT zv = VT(0);
   // stage A state (row ya): u behind / centre, face behind
   VT sc = opA(cl, a_first);
-  VT sb = in_mesh(a_first - dir) ? opA(cl, a_first - dir) : sc;
+  VT sb = in_mesh(a_first - dir) ? opA(cl, a_first - dir) : (ZE ? zv : sc);
   VT kb = ldv<NT, T, V>(Ky + face_b(a_first) + cl);
   T gb = 0, gc = 0, gkb = 0;
   if (ghost) {
     gc = opA1(gx, a_first);
-    gb = in_mesh(a_first - dir) ? opA1(gx, a_first - dir) : gc;
+    gb = in_mesh(a_first - dir) ? opA1(gx, a_first - dir) : (ZE ? T(0) : gc);
     gkb = Ky[face_b(a_first) + gx];
   }
   // stage B state (row yb): u' behind / at yb, u and u0 at yb, faces of yb
@@ -90,15 +93,15 @@ __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
     VT sa = sc, kx = VT(0), ka = kb, s0 = VT(0);
     T ga = gc, gka = gkb, gout = gc, gkl = 0, gkr = 0, ekr = 0, gs0 = 0;
     if (doA) {
-      sa = in_mesh(ya + dir) ? opA(cl, ya + dir) : sc;
+      sa = in_mesh(ya + dir) ? opA(cl, ya + dir) : (ZE ? zv : sc);
       kx = ldv<NT, T, V>(Kx + (long)ya * P + cl);
       ka = ldv<NT, T, V>(Ky + face_a(ya) + cl);
       s0 = src(cl, ya);
       if (ghost) {
-        ga = in_mesh(ya + dir) ? opA1(gx, ya + dir) : gc;
+        ga = in_mesh(ya + dir) ? opA1(gx, ya + dir) : (ZE ? T(0) : gc);
         gka = Ky[face_a(ya) + gx];
         const int go = ghost_l ? gx - 1 : gx + 1;  // the ghost's outer neighbour
-        gout = (go >= 0 && go < nx) ? opA1(go, ya) : gc;
+        gout = (go >= 0 && go < nx) ? opA1(go, ya) : (ZE ? T(0) : gc);
         gkl = Kx[(long)ya * P + gx];
         gkr = Kx[(long)ya * P + gx + 1];
         gs0 = src1(gx, ya);
@@ -136,7 +139,7 @@ __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
     if (doA) {
       if (ghost_l) leftA = gc;
       if (ghost_r) rightA = gc;
-      if (c == 0) leftA = sc[0];  // mirror at the left edge
+      if (c == 0) leftA = ZE ? T(0) : sc[0];  // the left edge
       const VT &vd = fwd ? sb : sa, &vu = fwd ? sa : sb;
       const VT &kyd = fwd ? kb : ka, &kyu = fwd ? ka : kb;
 #pragma unroll
@@ -144,8 +147,8 @@ __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
         const int col = c + e;
         const T l = (e == 0) ? leftA : sc[e - 1];
         T r;
-        if (e + 1 < V) r = (col + 1 < nx) ? sc[e + 1] : sc[e];
-        else r = (col + 1 < nx) ? rightA : sc[e];
+        if (e + 1 < V) r = (col + 1 < nx) ? sc[e + 1] : (ZE ? T(0) : sc[e]);
+        else r = (col + 1 < nx) ? rightA : (ZE ? T(0) : sc[e]);
         const T kr = (e + 1 < V) ? kx[e + 1] : kxrA;
         u1a[e] = jacobi_cell(s0[e], l, r, vd[e], vu[e], kx[e], kr, kyd[e], kyu[e], rx, ry);
       }
@@ -159,11 +162,12 @@ __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
     }
     // ---- stage B on row yb
     if (doB) {
-      const VT &u1d = in_mesh(yb - dir) ? u1b : u1c;  // behind yb (mirrored)
-      const VT &u1u = doA ? u1a : u1c;                // ahead of yb (mirrored)
+      // behind and ahead of yb (mirrored or zero outside the mesh)
+      const VT &u1d = in_mesh(yb - dir) ? u1b : (ZE ? zv : u1c);
+      const VT &u1u = doA ? u1a : (ZE ? zv : u1c);
       if (ghost_l) leftB = gu1c;
       if (ghost_r) rightB = gu1c;
-      if (c == 0) leftB = u1c[0];
+      if (c == 0) leftB = ZE ? T(0) : u1c[0];
       const VT &vd = fwd ? u1d : u1u, &vu = fwd ? u1u : u1d;
       const VT &kyd = fwd ? kbB : kaB, &kyu = fwd ? kaB : kbB;
       VT u2;
@@ -172,8 +176,8 @@ __device__ inline void walk_jacobi_pair(const Geo &g, const TileV<V> &tt,
         const int col = c + e;
         const T l = (e == 0) ? leftB : u1c[e - 1];
         T r;
-        if (e + 1 < V) r = (col + 1 < nx) ? u1c[e + 1] : u1c[e];
-        else r = (col + 1 < nx) ? rightB : u1c[e];
+        if (e + 1 < V) r = (col + 1 < nx) ? u1c[e + 1] : (ZE ? T(0) : u1c[e]);
+        else r = (col + 1 < nx) ? rightB : (ZE ? T(0) : u1c[e]);
         const T kr = (e + 1 < V) ? kxB[e + 1] : kxrB;
         u2[e] = jacobi_cell(srcB[e], l, r, vd[e], vu[e], kxB[e], kr, kyd[e], kyu[e], rx, ry);
       }

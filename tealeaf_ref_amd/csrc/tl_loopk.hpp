@@ -509,6 +509,197 @@ __global__ __launch_bounds__(BLOCK) void k_pp_ur(Geo G, int n, int first, double
   }
 }
 
+// PPCG outer iteration n, second pass with the block-Jacobi preconditioner:
+// as k_pp_ur, with z = M^-1 r by the 4-row Thomas solve of tea_block_solve
+// (kernels/tea_leaf_common_kernel.f90:378-468) over groups of 4 rows (tiles
+// start on multiples of 4 rows).
+template <bool NT, typename T, int V>
+__global__ __launch_bounds__(BLOCK) void k_pp_ur_blk(
+    Geo G, int n, int first, double rro_in, T *__restrict__ u, const T *__restrict__ p,
+    T *__restrict__ r, T *__restrict__ r_store, const T *__restrict__ w, T *__restrict__ z,
+    const T *__restrict__ Di, const T *__restrict__ Ky, const T *__restrict__ cp,
+    const T *__restrict__ bfp, T ry, const double *__restrict__ part_pw,
+    const double *__restrict__ part_zr, int nprev, Ctl *ctl, double *__restrict__ hrro,
+    double *__restrict__ halpha, int rev) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl->stop) return;
+  const double pw = dev::sum_partials(part_pw, nprev, lds);
+  const double rro = first ? rro_in : dev::sum_partials(part_zr, nprev, lds);
+  if (nonfinite_stop(n, pw, rro, ctl)) return;
+  const double alpha = rro / pw;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hrro[n] = rro;
+    halpha[n] = alpha;
+  }
+  const T a = (T)alpha;
+  const long P = G.pitch;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    if (tt.c >= G.nx) continue;
+    const int ngroups = (tt.y1 - tt.y0 + 3) / 4;
+    for (int m = 0; m < ngroups; ++m) {
+      const int g0 = tt.y0 + 4 * (rev ? ngroups - 1 - m : m);
+      const int nq = min(4, G.ny - g0);
+      VT rr[4], dp[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q >= nq) break;
+        const long i = (long)(g0 + q) * P + tt.c;
+        const VT uv = dev::ldv<NT, T, V>(u + i) + a * dev::ldv<NT, T, V>(p + i);
+        const VT ro = dev::ldv<NT, T, V>(r + i);
+        rr[q] = ro - a * dev::ldv<NT, T, V>(w + i);
+        dev::stv<false, T, V>(u + i - tt.c, tt.c, G.nx, uv);
+        dev::stv<false, T, V>(r_store + i - tt.c, tt.c, G.nx, ro);
+        dev::stv<false, T, V>(r + i - tt.c, tt.c, G.nx, rr[q]);
+      }
+      const long i0 = (long)g0 * P + tt.c;
+      dp[0] = rr[0] / dev::ldv<NT, T, V>(Di + i0);
+#pragma unroll
+      for (int q = 1; q < 4; ++q) {
+        if (q >= nq) break;
+        const long i = i0 + (long)q * P;
+        dp[q] = (rr[q] - (-dev::ldv<NT, T, V>(Ky + i) * ry) * dp[q - 1]) *
+                dev::ldv<NT, T, V>(bfp + i);
+      }
+      VT zn = dp[nq - 1];
+      for (int q = nq - 1; q >= 0; --q) {
+        const long i = i0 + (long)q * P;
+        const VT zq = q == nq - 1 ? dp[q] : dp[q] - dev::ldv<NT, T, V>(cp + i) * zn;
+        dev::stv<false, T, V>(z + i - tt.c, tt.c, G.nx, zq);
+        zn = zq;
+      }
+    }
+  }
+}
+
+// One inner sweep with the block-Jacobi preconditioner
+// (kernels/tea_leaf_ppcg_kernel.f90:269-285): rtemp -= A sd; z = M^-1 rtemp
+// (tea_block_solve); sd' = alpha sd + beta z; utemp += sd'.  The rows are
+// walked upwards on tiles that start on the 4-row blocks; the forward
+// elimination runs as each row arrives and, once the rows of a block have
+// been walked, the back substitution and the sd / utemp updates of the block
+// follow.  FIRST forms sd = src/theta, rtemp = r, utemp = sd on the fly
+// (tea_leaf_ppcg_init_sd); LAST writes z = utemp (update_z) and the
+// partials of (r - r_store).z and z.r, and nothing else (sd, rtemp and utemp
+// are dead after the last sweep of a fused solve).
+template <bool NT, typename T, int V, int FIRST, int LAST>
+__global__ __launch_bounds__(BLOCK) void k_pp_sweep_blk(
+    Geo G, int want_rrn, T theta_r, T alpha, T beta, const T *__restrict__ src,
+    const T *__restrict__ sdin, T *__restrict__ sdout, T *__restrict__ rtemp,
+    T *__restrict__ utemp, const T *__restrict__ r, const T *__restrict__ r_store,
+    T *__restrict__ z, const T *__restrict__ Di, const T *__restrict__ cp,
+    const T *__restrict__ bfp, const T *__restrict__ Kx, const T *__restrict__ Ky, T rx, T ry,
+    double *__restrict__ part_rrn, double *__restrict__ part_zr, const Ctl *ctl) {
+  typedef typename dev::Vec<T, V>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  if (ctl && ctl->stop) return;
+  const long P = G.pitch;
+  double arrn = 0.0, azr = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<V> tt = dev::tilev_of<V>(G, t);
+    auto ldop = [&](int col, int row) -> VT {
+      const long i = (long)row * P + col;
+      return FIRST ? dev::ldv<NT, T, V>(src + i) * theta_r : dev::ldv<NT, T, V>(sdin + i);
+    };
+    auto ld1 = [&](int col, int row) -> T {
+      const long i = (long)row * P + col;
+      return FIRST ? src[i] * theta_r : sdin[i];
+    };
+    struct Side {
+      VT rv, rt, ut, rs, ky, bfp, cp, di;
+    };
+    const int cc = tt.c <= G.nx ? tt.c : (G.nx / V) * V;
+    auto side = [&](int row) -> Side {
+      const long i = (long)row * P + cc;
+      const bool head = (row & 3) == 0;  // first row of a block: Di, else bfp
+      Side e;
+      e.rv = (FIRST || LAST) ? dev::ldv<NT, T, V>(r + i) : VT(0);
+      e.rt = FIRST ? VT(0) : dev::ldv<NT, T, V>(rtemp + i);
+      e.ut = FIRST ? VT(0) : dev::ldv<NT, T, V>(utemp + i);
+      e.rs = (LAST && want_rrn) ? dev::ldv<NT, T, V>(r_store + i) : VT(0);
+      e.ky = dev::ldv<false, T, V>(Ky + i);
+      e.cp = dev::ldv<NT, T, V>(cp + i);
+      e.bfp = head ? VT(0) : dev::ldv<NT, T, V>(bfp + i);
+      e.di = head ? dev::ldv<NT, T, V>(Di + i) : VT(0);
+      return e;
+    };
+    // the rows of the current block in walking order (shifted in): dp of
+    // the forward elimination, cp, sd and utemp before the sweep, r, r_store
+    VT dps[4], cps[4], sds[4], uts[4], rvs[4], rss[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dps[q] = cps[q] = sds[q] = uts[q] = rvs[q] = rss[q] = VT(0);
+    dev::walkv<NT, T, V, diff_form<T>(), TL_PF_SWEEP, TL_LDSX_SWEEP>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, side, [&](int y, VT v, VT wv, const Side &e) {
+      if (tt.c >= G.nx) return;
+      const VT rt = (FIRST ? e.rv : e.rt) - wv;
+      if (!LAST) dev::stv<false, T, V>(rtemp + (long)y * P, tt.c, G.nx, rt);
+      const int g0 = y & ~3;
+      const int nq = G.ny - g0 < 4 ? G.ny - g0 : 4;
+      const VT dp = (y == g0) ? rt / e.di : (rt - (-e.ky * ry) * dps[3]) * e.bfp;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        dps[q] = dps[q + 1];
+        cps[q] = cps[q + 1];
+        sds[q] = sds[q + 1];
+        uts[q] = uts[q + 1];
+        rvs[q] = rvs[q + 1];
+        rss[q] = rss[q + 1];
+      }
+      dps[3] = dp;
+      cps[3] = e.cp;
+      sds[3] = v;
+      uts[3] = FIRST ? v : e.ut;
+      rvs[3] = e.rv;
+      rss[3] = e.rs;
+      if (y - g0 + 1 < nq) return;
+      for (int k = nq; k < 4; ++k) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          dps[q] = dps[q + 1];
+          cps[q] = cps[q + 1];
+          sds[q] = sds[q + 1];
+          uts[q] = uts[q + 1];
+          rvs[q] = rvs[q + 1];
+          rss[q] = rss[q + 1];
+        }
+      }
+      VT zn = VT(0);
+#pragma unroll
+      for (int q = 3; q >= 0; --q) {
+        if (q >= nq) continue;
+        const VT zq = (q == nq - 1) ? dps[q] : dps[q] - cps[q] * zn;
+        zn = zq;
+        const VT sv = alpha * sds[q] + beta * zq;
+        const VT ut = uts[q] + sv;
+        const long ri = (long)(g0 + q) * P;
+        if (!LAST) {
+          dev::stv<false, T, V>(sdout + ri, tt.c, G.nx, sv);
+          dev::stv<false, T, V>(utemp + ri, tt.c, G.nx, ut);
+        } else {
+          dev::stv<false, T, V>(z + ri, tt.c, G.nx, ut);
+#pragma unroll
+          for (int e = 0; e < V; ++e)
+            if (tt.c + e < G.nx) {
+              if (want_rrn) arrn += ((double)rvs[q][e] - (double)rss[q][e]) * (double)ut[e];
+              azr += (double)ut[e] * (double)rvs[q][e];
+            }
+        }
+      }
+    }, 1);
+  }
+  if (LAST) {
+    arrn = dev::block_sum(arrn, lds);
+    azr = dev::block_sum(azr, lds);
+    if (threadIdx.x == 0) {
+      part_rrn[blockIdx.x] = arrn;
+      part_zr[blockIdx.x] = azr;
+    }
+  }
+}
+
 // One inner sweep of the polynomial preconditioner (kernels/tea_leaf_ppcg_kernel.f90:288-306):
 // rtemp -= A sd; sd' = alpha sd + beta (Mi) rtemp; utemp += sd'.  The first
 // sweep forms sd = src/theta, rtemp = r, utemp = sd on the fly
@@ -726,8 +917,8 @@ struct LoopBufs {
   T rx, ry;
   int diag;               // Jacobi-diagonal preconditioner
   const T *cg_src;        // r, or z when preconditioned: the CG direction source
-  // block-Jacobi preconditioner (CG only): Di, Thomas coefficients; G's tiles
-  // then start on multiples of 4 rows
+  // block-Jacobi preconditioner (FP64 CG and PPCG): Di, Thomas coefficients;
+  // G's tiles then start on multiples of 4 rows
   int block = 0;
   const T *Di = nullptr, *cp = nullptr, *bfp = nullptr;
 };
@@ -756,6 +947,33 @@ int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas
   const Geo G = b.G;
   const int nb = abs_blocks(G);
   const bool nt = use_nontemporal();
+  if constexpr (sizeof(T) == 8) {
+    if (b.block) {
+      // block preconditioner: one sweep per pass, walked upwards; the first
+      // reads z (M^-1 r from the update pass) and the last writes z, so they
+      // must be different passes (tl_ppcg_loop routes solves of one or two
+      // preconditioned sweeps to the per-routine path)
+      using KB = decltype(&k_pp_sweep_blk<true, T, V, 0, 0>);
+      static const KB kb[2][2][2] = {
+          {{&k_pp_sweep_blk<false, T, V, 0, 0>, &k_pp_sweep_blk<false, T, V, 0, 1>},
+           {&k_pp_sweep_blk<false, T, V, 1, 0>, &k_pp_sweep_blk<false, T, V, 1, 1>}},
+          {{&k_pp_sweep_blk<true, T, V, 0, 0>, &k_pp_sweep_blk<true, T, V, 0, 1>},
+           {&k_pp_sweep_blk<true, T, V, 1, 0>, &k_pp_sweep_blk<true, T, V, 1, 1>}}};
+      for (int i = 1; i <= inner; ++i) {
+        hipEvent_t ev;
+        prof_begin(2, &ev);
+        hipLaunchKernelGGL(kb[nt ? 1 : 0][i == 1 ? 1 : 0][i == inner ? 1 : 0], dim3(nb),
+                           dim3(BLOCK), 0, s.stream, G, want_rrn, theta_r, (T)alphas[i - 1],
+                           (T)betas[i - 1], b.z, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r,
+                           b.r_store, b.z, b.Di, b.cp, b.bfp, b.Kx, b.Ky, b.rx, b.ry,
+                           m.part(P3), m.part(P2), ctl);
+        prof_end(2, ev);
+        sd = 1 - sd;
+      }
+      TL_HIP(hipGetLastError());
+      return TL_OK;
+    }
+  }
   auto k1 = nt ? k_pp_sweep<true, T, V> : k_pp_sweep<false, T, V>;
   const T *src = b.diag ? b.z : b.r;
   const bool pairs = sweep_pairs();
@@ -900,9 +1118,21 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
                        m.part(P3), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, d);
     prof_end(5, ev);
     prof_begin(6, &ev);
-    hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, rro_first,
-                       b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag, m.part(P1),
-                       m.part(P2), nb, m.ctl, m.hrro, m.halpha, d > 0 ? 1 : 0);
+    bool block_ur = false;
+    if constexpr (sizeof(T) == 8) {
+      if (b.block) {
+        auto kub = nt ? k_pp_ur_blk<true, T, V> : k_pp_ur_blk<false, T, V>;
+        hipLaunchKernelGGL(kub, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
+                           rro_first, b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Di, b.Ky,
+                           b.cp, b.bfp, b.ry, m.part(P1), m.part(P2), nb, m.ctl, m.hrro,
+                           m.halpha, d > 0 ? 1 : 0);
+        block_ur = true;
+      }
+    }
+    if (!block_ur)
+      hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
+                         rro_first, b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag,
+                         m.part(P1), m.part(P2), nb, m.ctl, m.hrro, m.halpha, d > 0 ? 1 : 0);
     prof_end(6, ev);
     T *rt_cur = b.rtemp;
     return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, m.ctl, m, sd, rt_cur, d);

@@ -85,11 +85,12 @@ __global__ __launch_bounds__(BLOCK) void k_jl(Geo G, int n, int first, double ep
 }
 
 // The same iteration on the vector walk (two columns per lane, 16-byte loads,
-// wave shuffles and LDS for the neighbours), for chunks whose external faces
-// have zero coefficients (not reflective), where the value read outside the
-// mesh is multiplied by zero (the walk mirrors, the per-cell kernel reads the
-// zero halo of un).
-template <bool NT>
+// wave shuffles and LDS for the neighbours), for chunks with every face
+// external.  Outside the mesh: ZE = false mirrors (the faces' coefficients
+// are zero, so the value is multiplied by zero), ZE = true reads zero
+// (reflective faces: the reference reads the halo of un, which it never
+// writes, as the per-cell kernel's ZERO walk does).
+template <bool NT, bool ZE>
 __global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double eps_abs,
                                                 const double *__restrict__ un,
                                                 double *__restrict__ uout,
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double 
       return (s0[e] + rx * (kxr * r + kxl * l) + ry * (kyu * u + kyd * d)) /
              (1.0 + rx * (kxl + kxr) + ry * (kyd + kyu));
     };
-    dev::walkv<NT, double, 2, false, TL_PF_W, true>(
+    dev::walkv<NT, double, 2, false, TL_PF_W, true, ZE>(
         G, tt, Kx, Ky, rx, ry, ldop, ld1, src, [&](int y, VT v, VT w, const VT &) {
       if (tt.c >= G.nx) return;
       dev::stv<false, double, 2>(uout + (long)y * P, tt.c, G.nx, w);
@@ -177,7 +178,7 @@ __device__ inline bool jacobi_pair_exit(int n, double eps_abs, const double *__r
 #define TL_JACOBI_PAIR_WAVES 4
 #endif
 
-template <bool NT>
+template <bool NT, bool ZE>
 __global__ __launch_bounds__(BLOCK, TL_JACOBI_PAIR_WAVES) void k_jl_pair(
     Geo G, int n, int first, double eps_abs, const double *__restrict__ un,
     double *__restrict__ umid, double *__restrict__ uout, const double *__restrict__ u0,
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(BLOCK, TL_JACOBI_PAIR_WAVES) void k_jl_pair(
       return dev::ldv<NT, double, 2>(u0 + (long)row * P + col);
     };
     auto src1 = [&](int col, int row) -> double { return u0[(long)row * P + col]; };
-    dev::walk_jacobi_pair<NT, 2>(G, tt, Kx, Ky, rx, ry, ldop, ld1, src, src1,
+    dev::walk_jacobi_pair<NT, 2, ZE>(G, tt, Kx, Ky, rx, ry, ldop, ld1, src, src1,
                                  [&](int y, VT v0, VT v1, VT v2) {
       if (tt.c >= G.nx) return;
       const long row = (long)y * P;
@@ -432,13 +433,166 @@ __global__ __launch_bounds__(BLOCK, TL_CHEBY_PAIR_WAVES) void k_cheby_pair(
 }  // namespace
 
 namespace tl {
+// One Chebyshev iteration with the block-Jacobi preconditioner in one pass
+// (kernels/tea_leaf_cheby_kernel.f90:111-200 with tea_block_solve,
+// kernels/tea_leaf_common_kernel.f90:378-468): w = A u and r = u0 - w row by
+// row, the forward elimination of the 4-row Thomas solve as each row
+// arrives, and once the rows of a preconditioner block have been walked the
+// back substitution z = M^-1 r, p = alpha p + beta z and u' = u + p for the
+// block's rows.  Tiles start on the blocks (multiples of 4 rows) and are
+// walked upwards.  LAST (the last iteration of a run, before a residual
+// check) also stores w, r and z and the partials of r.r.
+template <bool NT, int LAST>
+__global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_blk(
+    Geo G, const double *__restrict__ uin, double *__restrict__ uout,
+    const double *__restrict__ u0, const double *__restrict__ Kx, const double *__restrict__ Ky,
+    double *__restrict__ p, double *__restrict__ w, double *__restrict__ r,
+    double *__restrict__ z, const double *__restrict__ Di, const double *__restrict__ cp,
+    const double *__restrict__ bfp, double rx, double ry, double alpha, double beta,
+    double *__restrict__ out_part) {
+  typedef typename dev::Vec<double, 2>::type VT;
+  __shared__ double lds[BLOCK / WAVE];
+  const long P = G.pitch;
+  double acc = 0.0;
+  dev::TileIter it(G.tiles, G.blocks);
+  for (long t = it.t; t < it.t_end; ++t) {
+    const dev::TileV<2> tt = dev::tilev_of<2>(G, t);
+    const int cc = tt.c <= G.nx ? tt.c : (G.nx / 2) * 2;
+    struct Side {
+      VT u0, p, ky, bfp, cp, di;
+    };
+    auto side = [&](int row) -> Side {
+      const long i = (long)row * P + cc;
+      Side e;
+      e.u0 = dev::ldv<NT, double, 2>(u0 + i);
+      e.p = dev::ldv<NT, double, 2>(p + i);
+      e.ky = dev::ldv<false, double, 2>(Ky + i);
+      e.cp = dev::ldv<NT, double, 2>(cp + i);
+      const bool head = (row & 3) == 0;  // first row of a block: Di, else bfp
+      e.bfp = head ? VT(0) : dev::ldv<NT, double, 2>(bfp + i);
+      e.di = head ? dev::ldv<NT, double, 2>(Di + i) : VT(0);
+      return e;
+    };
+    auto ldop = [&](int col, int row) -> VT {
+      return dev::ldv<NT, double, 2>(uin + (long)row * P + col);
+    };
+    auto ld1 = [&](int col, int row) -> double { return uin[(long)row * P + col]; };
+    // the rows of the current block in walking order (shifted in), with
+    // dp of the forward elimination, cp, p and u of each row
+    VT dps[4], cps[4], pps[4], uus[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dps[q] = cps[q] = pps[q] = uus[q] = VT(0);
+    dev::walkv<NT, double, 2, false, TL_PF_CHEBY, true>(
+        G, tt, Kx, Ky, rx, ry, ldop, ld1, side, [&](int y, VT v, VT wv, const Side &e) {
+      if (tt.c >= G.nx) return;
+      const long row = (long)y * P;
+      const VT rv = e.u0 - wv;
+      const int g0 = y & ~3;
+      const int nq = G.ny - g0 < 4 ? G.ny - g0 : 4;
+      // tea_block_solve's forward elimination, in its order of operations
+      const VT dp = (y == g0) ? rv / e.di : (rv - (-e.ky * ry) * dps[3]) * e.bfp;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        dps[q] = dps[q + 1];
+        cps[q] = cps[q + 1];
+        pps[q] = pps[q + 1];
+        uus[q] = uus[q + 1];
+      }
+      dps[3] = dp;
+      cps[3] = e.cp;
+      pps[3] = e.p;
+      uus[3] = v;
+      if (LAST) {
+        dev::stv<false, double, 2>(w + row, tt.c, G.nx, wv);
+        dev::stv<false, double, 2>(r + row, tt.c, G.nx, rv);
+        acc += rv.x * rv.x;
+        if (tt.c + 1 < G.nx) acc += rv.y * rv.y;
+      }
+      // carry the halo ring of u to the new buffer
+      if (tt.c == 0) uout[row - 1] = uin[row - 1];
+      if (tt.c + 2 >= G.nx) uout[row + G.nx] = uin[row + G.nx];
+      for (int side_row = 0; side_row < 2; ++side_row) {
+        if (y != (side_row ? G.ny - 1 : 0)) continue;
+        const long hr = side_row ? (long)G.ny * P : -P;
+        uout[hr + tt.c] = uin[hr + tt.c];
+        if (tt.c + 1 < G.nx) uout[hr + tt.c + 1] = uin[hr + tt.c + 1];
+      }
+      if (y - g0 + 1 < nq) return;
+      // the block is complete: its rows sit in slots 4 - nq .. 3; move them
+      // to 0 .. nq - 1, then substitute back from the top row
+      for (int k = nq; k < 4; ++k) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          dps[q] = dps[q + 1];
+          cps[q] = cps[q + 1];
+          pps[q] = pps[q + 1];
+          uus[q] = uus[q + 1];
+        }
+      }
+      VT zn = VT(0);
+#pragma unroll
+      for (int q = 3; q >= 0; --q) {
+        if (q >= nq) continue;
+        const VT zq = (q == nq - 1) ? dps[q] : dps[q] - cps[q] * zn;
+        const VT pv = alpha * pps[q] + beta * zq;
+        const long ri = (long)(g0 + q) * P;
+        dev::stv<false, double, 2>(p + ri, tt.c, G.nx, pv);
+        dev::stv<false, double, 2>(uout + ri, tt.c, G.nx, uus[q] + pv);
+        if (LAST) dev::stv<false, double, 2>(z + ri, tt.c, G.nx, zq);
+        zn = zq;
+      }
+    }, 1);
+  }
+  acc = dev::block_sum(acc, lds);
+  if (LAST && threadIdx.x == 0) out_part[blockIdx.x] = acc;
+}
+
+// Chebyshev iterations with the block preconditioner, one k_cheby_blk pass
+// each (cheby_run_fused's contract).
+int cheby_run_block(const double *ch_alphas, const double *ch_betas, int step_first, int count,
+                    int want_norm, double *norm) {
+  State &s = S();
+  Geo G = geo(2 * BLOCK);
+  // tiles start on the 4-row blocks of the preconditioner
+  const int blocks = G.blocks < 0 ? -G.blocks : G.blocks;
+  G.rows_per_tile = (G.rows_per_tile + 3) / 4 * 4;
+  G.tiles = (long)G.tiles_x * ((G.ny + G.rows_per_tile - 1) / G.rows_per_tile);
+  const int nb = (int)(G.tiles < blocks ? G.tiles : blocks);
+  G.blocks = G.blocks < 0 ? -nb : nb;
+  const bool nt = use_nontemporal();
+  for (int i = 0; i < count; ++i) {
+    const int st = step_first + i - 1;
+    const int in = s.ucur, outb = 1 - s.ucur;
+    const bool last = i + 1 == count;
+    auto k = nt ? (last ? k_cheby_blk<true, 1> : k_cheby_blk<true, 0>)
+                : (last ? k_cheby_blk<false, 1> : k_cheby_blk<false, 0>);
+    hipEvent_t ev;
+    prof_begin(3, &ev);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
+                       s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
+                       s.Di.p11, s.cp.p11, s.bfp.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st],
+                       part(P0));
+    prof_end(3, ev);
+    if (hipGetLastError() != hipSuccess) return -fail(TL_ERR_HIP, "Chebyshev pass launch");
+    s.ucur = outb;
+  }
+  if (want_norm && count > 0) {
+    const int slot = P0;
+    const int rc = finish_sums(&slot, 1, nb, norm);
+    if (rc) return -rc;
+  }
+  return 1;
+}
+
 // Chebyshev iterations through the vector walk; returns 1 if it ran them,
 // 0 if the configuration needs the per-routine kernel (reflective faces,
-// block preconditioner, internal faces), or a negative error.
+// internal faces), or a negative error.
 int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_first, int count,
                     int want_norm, double *norm) {
   State &s = S();
-  if (s.reflective || !s.zb_all || s.precond == TL_PREC_JAC_BLOCK) return 0;
+  if (s.reflective || !s.zb_all) return 0;
+  if (s.precond == TL_PREC_JAC_BLOCK) return cheby_run_block(ch_alphas, ch_betas, step_first,
+                                                             count, want_norm, norm);
   const Geo G = geo_resident(2 * BLOCK, TL_CHEBY_PAIR_WAVES);
   const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int diag = s.precond == TL_PREC_JAC_DIAG;
@@ -600,8 +754,8 @@ int cg_loop_per_routine(tl_loop_state *st, double *cg_alphas, double *cg_betas,
   return TL_OK;
 }
 
-// PPCG with the block preconditioner (or no inner steps): the per-routine
-// path (tea_solve.f90:294-330), one outer iteration at a time.
+// PPCG without inner steps, or preconditioned with one or two: the
+// per-routine path (tea_solve.f90:294-330), one outer iteration at a time.
 int ppcg_loop_per_routine(tl_loop_state *st, const double *alphas, const double *betas, int inner,
                           double theta, int *inner_iters, std::vector<double> &errs) {
   int rc;
@@ -811,7 +965,11 @@ extern "C" int tl_ppcg_loop(tl_loop_state *st, const double *alphas, const doubl
   if (rc) return rc;
   if (!alphas || !betas) return fail(TL_ERR_ARG, "tl_ppcg_loop: null coefficients");
   State &s = S();
-  if (s.precond == TL_PREC_JAC_BLOCK || inner < 1)
+  // per routine: no inner steps, or a preconditioner with one or two inner
+  // steps, where the fused pass that reads z (M^-1 r) as the first sweep's
+  // operand would also write z as the last sweep's result, racing with the
+  // neighbouring tiles' reads
+  if (inner < 1 || (s.precond != TL_PREC_NONE && inner <= 2))
     return with_errors(st, [&](std::vector<double> &errs) {
       return ppcg_loop_per_routine(st, alphas, betas, inner, theta, inner_iters, errs);
     });
@@ -841,8 +999,11 @@ int jacobi_loop_pairs(tl_loop_state *st) {
   Ctl *ctl = (Ctl *)s.ctl;
   double *herr = loop_mem().herr;
   const bool nt = use_nontemporal();
-  auto kp = nt ? k_jl_pair<true> : k_jl_pair<false>;
-  auto kv = nt ? k_jl_v<true> : k_jl_v<false>;
+  const bool ze = s.reflective;
+  auto kp = nt ? (ze ? k_jl_pair<true, true> : k_jl_pair<true, false>)
+               : (ze ? k_jl_pair<false, true> : k_jl_pair<false, false>);
+  auto kv = nt ? (ze ? k_jl_v<true, true> : k_jl_v<true, false>)
+               : (ze ? k_jl_v<false, true> : k_jl_v<false, false>);
   const int n_first = st->n_first, n_last = st->n_last;
   auto slot_a = [](int j) { return part((j & 1) ? P4 : P2); };
   auto slot_b = [](int j) { return part((j & 1) ? P5 : P3); };
@@ -916,17 +1077,19 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   if (rc) return rc;
   if ((rc = ensure_loop_buffers(st->n_last))) return rc;
   State &s = S();
-  // the vector walk where every external face coefficient is zero (all faces
-  // external, not reflective); the per-cell walk otherwise, which reads the
-  // halo of un as the reference does (never written: zero)
-  const bool vec = s.zb_all && !s.reflective;
+  // the vector walk where every face is external (reflective faces read zero
+  // outside the mesh, as the reference reads the never-written halo of un);
+  // the per-cell walk otherwise
+  const bool vec = s.zb_all;
   if (vec && sweep_pairs()) return jacobi_loop_pairs(st);
   const Geo G = vec ? geo(2 * BLOCK) : geo(BLOCK);
   const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int u0c = s.ucur;
   Ctl *ctl = (Ctl *)s.ctl;
   double *herr = loop_mem().herr;
-  auto kv = use_nontemporal() ? k_jl_v<true> : k_jl_v<false>;
+  const bool ze = s.reflective;
+  auto kv = use_nontemporal() ? (ze ? k_jl_v<true, true> : k_jl_v<true, false>)
+                              : (ze ? k_jl_v<false, true> : k_jl_v<false, false>);
   auto launch = [&](int n, bool first) -> int {
     const int in = u0c ^ ((n - st->n_first) & 1);
     hipEvent_t ev;

@@ -112,8 +112,9 @@ struct ApplyA {};
 // same block are exchanged through LDS (one barrier per row) instead of being
 // loaded again; only the block's outer edges are loaded.  All waves of the
 // block must walk the same rows.
-template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, typename LDV,
-          typename LD1, typename EXTRA, typename BODY, typename CELL = ApplyA>
+// Outside the mesh the operand is mirrored (ZE = false) or zero (ZE = true).
+template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, bool ZE = false,
+          typename LDV, typename LD1, typename EXTRA, typename BODY, typename CELL = ApplyA>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
                              EXTRA extra, BODY body, int dir = 1, CELL cell = CELL()) {
@@ -142,8 +143,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
   // left and right of the wave; Kx right of the wave; the extra data
   VT vc = ldv_op(cl, ys);
   const int yb = ys - dir, ya = ys + dir;
-  VT vb = (yb >= 0 && yb < ny) ? ldv_op(cl, yb) : vc;
-  VT va = (ya >= 0 && ya < ny) ? ldv_op(cl, ya) : vc;
+  VT vb = (yb >= 0 && yb < ny) ? ldv_op(cl, yb) : (ZE ? VT(0) : vc);
+  VT va = (ya >= 0 && ya < ny) ? ldv_op(cl, ya) : (ZE ? VT(0) : vc);
   VT kb = ldv<NT, T, V>(Ky + (long)(fwd ? ys : ys + 1) * P + cl);
   VT ka = ldv<NT, T, V>(Ky + face_ahead(ys));
   VT kx = ldv<NT, T, V>(Kx + (long)ys * P + cl);
@@ -160,7 +161,7 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     E ex2 = ex;
     auto load_next = [&]() {
       const int yna = yn + dir;
-      va2 = (yna >= 0 && yna < ny) ? ldv_op(cl, yna) : va;
+      va2 = (yna >= 0 && yna < ny) ? ldv_op(cl, yna) : (ZE ? VT(0) : va);
       ka2 = ldv<NT, T, V>(Ky + face_ahead(yn));
       kx2 = ldv<NT, T, V>(Kx + (long)yn * P + cl);
       if (need_left) el2 = ld1_op(tt.x0 - 1, yn);
@@ -189,7 +190,7 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     if (need_left) left = el;
     if (need_right) right = er;
     if (need_kxr) kxr = ek;
-    if (c == 0) left = vc[0];  // mirror at the left edge
+    if (c == 0) left = ZE ? T(0) : vc[0];  // the left edge
     const VT &vd = fwd ? vb : va, &vu = fwd ? va : vb;
     const VT &kyd = fwd ? kb : ka, &kyu = fwd ? ka : kb;
     VT w;
@@ -198,8 +199,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
       const int col = c + e;
       const T l = (e == 0) ? left : vc[e - 1];
       T r;
-      if (e + 1 < V) r = (col + 1 < nx) ? vc[e + 1] : vc[e];
-      else r = (col + 1 < nx) ? right : vc[e];
+      if (e + 1 < V) r = (col + 1 < nx) ? vc[e + 1] : (ZE ? T(0) : vc[e]);
+      else r = (col + 1 < nx) ? right : (ZE ? T(0) : vc[e]);
       const T kl = kx[e];
       const T kr = (e + 1 < V) ? kx[e + 1] : kxr;
       if constexpr (std::is_same<CELL, ApplyA>::value)

@@ -56,6 +56,12 @@ CASES = {
                              max_iters=1500, end_step=3),
     "ppcg_jacblock_96": dict(cells=96, solver="ppcg", seed=13, precond="jac_block", inner_steps=6,
                              end_step=3),
+    "cheby_jacblock_98": dict(cells=98, solver="chebyshev", seed=17, precond="jac_block",
+                              eps=1e-10, max_iters=1500, end_step=3),
+    # capped: every step runs 200 iterations, so the summaries pin the
+    # arithmetic of the reflective faces without a rounding-decided exit
+    "jacobi_reflective_66": dict(cells=66, solver="jacobi", seed=19, reflective=True, eps=1e-15,
+                                 max_iters=200, end_step=3),
     "cg_checkresult_64": dict(cells=64, solver="cg", seed=12, check_result=True, end_step=2),
     # small decks with the profiler on whose whole tea.out layout is kept
     "layout_cg_32": dict(cells=32, solver="cg", seed=14, precond="jac_diag", check_result=True,

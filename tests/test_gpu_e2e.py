@@ -1,7 +1,7 @@
 """End to end: the native host (Fortran, in process) runs each reference deck on
 the GPU; its tea.out must reproduce the reference's per-step `step:` rows
 within 1e-10 relative and its iteration counts (tea_solve.f90:448-453):
-exactly on 119 of the 132 steps the fixtures hold.  The other 13 are exits
+exactly on 126 of the 142 steps the fixtures hold.  The other 16 are exits
 decided by rounding (fixtures_util.iteration_bounds): the reference and the
 oracle built with multiply-add contraction disagree there, or, at tl_eps
 1e-15, the reference's exit error lies within 2 % of the threshold; there

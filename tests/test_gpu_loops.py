@@ -101,12 +101,16 @@ def test_cg_loop_switch_and_resume(pair_factory):
     p.cmp("u", tol=1e-10)
 
 
-@pytest.mark.parametrize("precond", [1, 2])
+# (preconditioner, inner steps): the fused sweeps (two per pass, or one per
+# pass with the block preconditioner, an odd count ending on a single one),
+# and the per-routine path of a preconditioned solve with two inner steps
+@pytest.mark.parametrize("precond,inner", [(1, 6), (2, 6), (3, 6), (3, 5), (2, 2), (3, 2)])
 @pytest.mark.parametrize("reflective", [False, True])
-def test_ppcg_loop(pair_factory, precond, reflective):
+def test_ppcg_loop(pair_factory, precond, inner, reflective):
+    print(f"ppcg loop precond={precond} inner={inner} reflective={reflective}", flush=True)
     p = pair_factory(77, 58, 1, seed=31, precond=precond, reflective=reflective)
     r0, rro = _setup(p)
-    eigmin, eigmax, inner = 0.9, 40.0, 6
+    eigmin, eigmax = 0.9, 40.0
     ch_a, ch_b, theta = O.calc_ch_coefs(eigmin, eigmax, inner)
     # the switch block (tea_solve.f90:229-250) on both sides
     p.o.ppcg_init(2, precond, 1)
@@ -206,6 +210,34 @@ def test_cheby_run(pair_factory):
            b.ctypes.data_as(C.POINTER(C.c_double)), 1, 20, 1, C.byref(rr))
     assert_scalar(rr.value, rr_o, 1e-8, "cheby residual")
     p.cmp("u", tol=1e-10)
+
+
+@pytest.mark.parametrize("nx,ny,count", [(64, 40, 20), (64, 42, 19), (1100, 70, 11), (37, 5, 6),
+                                         (530, 259, 8)])
+def test_cheby_run_block(pair_factory, nx, ny, count):
+    """Chebyshev iterations with the block-Jacobi preconditioner fused into
+    one pass each (k_cheby_blk: the 4-row Thomas solve on the rows as the
+    walk passes them) against the oracle's cheby_iterate with
+    tea_block_solve, including meshes whose height is not a multiple of the
+    4-row blocks and the fields the last pass of a run stores."""
+    print(f"cheby block {nx}x{ny} count={count}", flush=True)
+    p = pair_factory(nx, ny, 1, seed=47, precond=L.PREC_JAC_BLOCK)
+    _setup(p)
+    theta = 20.0
+    ch_a, ch_b, _ = O.calc_ch_coefs(1.0, 39.0, count + 2)
+    p.o.cheby_init(theta, L.PREC_JAC_BLOCK)
+    L.call("tl_cheby_init", theta)
+    for s_ in range(1, count + 1):
+        p.o.cheby_iterate(ch_a[s_ - 1], ch_b[s_ - 1], L.PREC_JAC_BLOCK)
+    rr_o = p.o.calc_2norm(1)
+    a = np.ascontiguousarray(ch_a)
+    b = np.ascontiguousarray(ch_b)
+    rr = C.c_double()
+    L.call("tl_cheby_run", a.ctypes.data_as(C.POINTER(C.c_double)),
+           b.ctypes.data_as(C.POINTER(C.c_double)), 1, count, 1, C.byref(rr))
+    assert_scalar(rr.value, rr_o, 1e-8, "cheby residual")
+    for f in ("u", "p", "w", "r", "z"):
+        p.cmp(f, tol=1e-10)
 
 
 @pytest.mark.parametrize("precond,count,nx,ny", [(1, 20, 1100, 70), (2, 21, 1100, 70),
@@ -320,19 +352,23 @@ def test_profile_sampling():
     assert ms1 > 0.0 and ms4 > 0.0
 
 
-@pytest.mark.parametrize("nx,ny", [(45, 50), (1100, 70), (300, 1400)])
-def test_jacobi_pairs_match_singles(nx, ny, monkeypatch):
+@pytest.mark.parametrize("nx,ny,reflective", [(45, 50, False), (1100, 70, False),
+                                               (300, 1400, False), (45, 50, True),
+                                               (1100, 70, True)])
+def test_jacobi_pairs_match_singles(nx, ny, reflective, monkeypatch):
     """Two Jacobi iterations per pass (with both exit tests per pass, the
     first iteration's u kept for an exit there) give what one iteration per
     pass gives: the same exit iteration and reason, u to the last bits, for
-    exits at either iteration of a pair and for odd and even limits."""
+    exits at either iteration of a pair and for odd and even limits; with
+    reflective faces both read zero outside the mesh."""
     cases = [(1e-3, 499), (3e-4, 499), (1e-30, 7), (1e-30, 8), (1e-30, 1)]
     got = {}
     for pairs in ("0", "1"):
         monkeypatch.setenv("TL_SWEEP_PAIRS", pairs)
         for eps_abs, n_last in cases:
-            print(f"jacobi pairs={pairs} {nx}x{ny} eps {eps_abs} n_last {n_last}", flush=True)
-            p = Pair(nx, ny, 1, seed=53)
+            print(f"jacobi pairs={pairs} {nx}x{ny} reflective={reflective} eps {eps_abs} "
+                  f"n_last {n_last}", flush=True)
+            p = Pair(nx, ny, 1, seed=53, reflective=reflective)
             try:
                 p.init()
                 st = L.LoopState(n_first=1, n_last=n_last, eps_abs=eps_abs)
