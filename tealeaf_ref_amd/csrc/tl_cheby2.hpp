@@ -122,9 +122,9 @@ __device__ inline void walk_cheby_pair(const Geo &g, const TileV<V> &tt,
       xch[par][wave][4] = u1c[V - 1];
     }
     __syncthreads();
-    T leftA = __shfl_up(sc[V - 1], 1, WAVE), rightA = __shfl_down(sc[0], 1, WAVE);
-    T kxrA = __shfl_down(kx[0], 1, WAVE);
-    T leftB = __shfl_up(u1c[V - 1], 1, WAVE), rightB = __shfl_down(u1c[0], 1, WAVE);
+    T leftA = from_lower_lane(sc[V - 1]), rightA = from_upper_lane(sc[0]);
+    T kxrA = from_upper_lane(kx[0]);
+    T leftB = from_lower_lane(u1c[V - 1]), rightB = from_upper_lane(u1c[0]);
     if (lane == 0 && wave > 0) {
       leftA = xch[par][wave - 1][1];
       leftB = xch[par][wave - 1][4];

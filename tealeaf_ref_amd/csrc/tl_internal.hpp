@@ -159,6 +159,49 @@ int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_fi
 namespace tl {
 namespace dev {
 
+// The value of the next lower (lane - 1) or next higher (lane + 1) lane of
+// the wave; lane 0 (lane 63) gets its own value back, as __shfl_up /
+// __shfl_down(v, 1) give it.  TL_DPP = 1 moves it with a DPP wave shift
+// (v_mov_b32 with wave_shr:1 / wave_shl:1, one VALU instruction per 32 bits:
+// lane i takes lane i - 1 / i + 1, the lane without a source keeps `old`),
+// TL_DPP = 0 through the LDS crossbar (ds_bpermute): the default, as the DPP
+// form made the CG w pass 8 % slower at 4096^2 (135 against 125 us) and
+// changed no other pass by more than 1 %.
+#ifndef TL_DPP
+#define TL_DPP 0
+#endif
+template <int CTRL>
+__device__ inline unsigned dpp32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL, typename T>
+__device__ inline T dpp_shift(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = dpp32<CTRL>((unsigned)b), hi = dpp32<CTRL>((unsigned)(b >> 32));
+    return __builtin_bit_cast(T, (unsigned long long)lo | ((unsigned long long)hi << 32));
+  } else {
+    return __builtin_bit_cast(T, dpp32<CTRL>(__builtin_bit_cast(unsigned, v)));
+  }
+}
+template <typename T>
+__device__ inline T from_lower_lane(T v) {
+#if TL_DPP
+  return dpp_shift<0x138>(v);  // wave_shr:1
+#else
+  return __shfl_up(v, 1, WAVE);
+#endif
+}
+template <typename T>
+__device__ inline T from_upper_lane(T v) {
+#if TL_DPP
+  return dpp_shift<0x130>(v);  // wave_shl:1
+#else
+  return __shfl_down(v, 1, WAVE);
+#endif
+}
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);

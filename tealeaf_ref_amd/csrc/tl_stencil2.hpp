@@ -94,9 +94,9 @@ __device__ inline void walk2(const Geo &g, const Tile2 &tt, const double *__rest
     const d2 kyu = ld2<NT>(Ky + (long)(y + 1) * P + cl);
     const d2 kx = ld2<NT>(Kx + (long)y * P + cl);
     // column neighbours of the pair from the adjacent lanes
-    double left = __shfl_up(vc.y, 1, WAVE);
-    double right = __shfl_down(vc.x, 1, WAVE);
-    double kxr = __shfl_down(kx.x, 1, WAVE);
+    double left = from_lower_lane(vc.y);
+    double right = from_upper_lane(vc.x);
+    double kxr = from_upper_lane(kx.x);
     if (need_left) left = ld1p(tt.x0 - 1, y);
     if (need_right) right = ld1p(xr, y);
     if (need_kxr) kxr = ld1<NT>(Kx + (long)y * P + xr);

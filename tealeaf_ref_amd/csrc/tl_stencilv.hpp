@@ -170,9 +170,9 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
       ex2 = extra(yn);
     };
     if (PF && more) load_next();
-    T left = __shfl_up(vc[V - 1], 1, WAVE);
-    T right = __shfl_down(vc[0], 1, WAVE);
-    T kxr = __shfl_down(kx[0], 1, WAVE);
+    T left = from_lower_lane(vc[V - 1]);
+    T right = from_upper_lane(vc[0]);
+    T kxr = from_upper_lane(kx[0]);
     if (LDSX) {
       const int par = k & 1;
       if (lane == 0) {
