@@ -121,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_jl_v(Geo G, int n, int first, double 
       return (s0[e] + rx * (kxr * r + kxl * l) + ry * (kyu * u + kyd * d)) /
              (1.0 + rx * (kxl + kxr) + ry * (kyd + kyu));
     };
-    dev::walkv<NT, double, 2, false, TL_PF_W, true, ZE>(
+    dev::walkv<NT, double, 2, false, TL_PF_W, true, ZE ? 1 : 0>(
         G, tt, Kx, Ky, rx, ry, ldop, ld1, src, [&](int y, VT v, VT w, const VT &) {
       if (tt.c >= G.nx) return;
       dev::stv<false, double, 2>(uout + (long)y * P, tt.c, G.nx, w);
@@ -267,8 +267,12 @@ __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev
 // u' = u + p into the other u buffer.  w, r (and z) are stored, and the
 // partials of r.r written, only on the last iteration of a run: nothing reads
 // them in between.  The halo ring of u is carried over to the new buffer, as
-// the per-routine kernel does.  For chunks whose external faces are zero
-// (not reflective), where the halo values never enter the operator.
+// the per-routine kernel does.  EDGE 0: chunks whose external faces are zero
+// (not reflective), where the mirrored values outside the mesh are
+// multiplied by zero; EDGE 2: reflective faces, whose nonzero coefficients
+// take the halo cells of u as they are in memory, as the reference does (it
+// refreshes only p's halo during the Chebyshev iterations, so u's halo
+// stays as the first Chebyshev step left it, carried from buffer to buffer).
 // ---------------------------------------------------------------------------
 #ifndef TL_PF_CHEBY
 #define TL_PF_CHEBY 0
@@ -278,7 +282,7 @@ __global__ void k_jl_tail(int n, double eps_abs, const double *__restrict__ prev
 #define TL_CHEBY_WAVES 1
 #endif
 
-template <bool NT>
+template <bool NT, int EDGE>
 __global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_v(Geo G, const double *__restrict__ uin,
                                                     double *__restrict__ uout,
                                                     const double *__restrict__ u0,
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(BLOCK, TL_CHEBY_WAVES) void k_cheby_v(Geo G, const 
       return dev::ldv<NT, double, 2>(uin + (long)row * P + col);
     };
     auto ld1 = [&](int col, int row) -> double { return uin[(long)row * P + col]; };
-    dev::walkv<NT, double, 2, false, TL_PF_CHEBY, true>(
+    dev::walkv<NT, double, 2, false, TL_PF_CHEBY, true, EDGE>(
         G, tt, Kx, Ky, rx, ry, ldop, ld1, side, [&](int y, VT v, VT wv, const Side &e) {
       if (tt.c >= G.nx) return;
       const long row = (long)y * P;
@@ -585,26 +589,30 @@ int cheby_run_block(const double *ch_alphas, const double *ch_betas, int step_fi
 }
 
 // Chebyshev iterations through the vector walk; returns 1 if it ran them,
-// 0 if the configuration needs the per-routine kernel (reflective faces,
-// internal faces), or a negative error.
+// 0 if the configuration needs the per-routine kernel (internal faces, or
+// reflective faces with the block preconditioner), or a negative error.
 int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_first, int count,
                     int want_norm, double *norm) {
   State &s = S();
-  if (s.reflective || !s.zb_all) return 0;
-  if (s.precond == TL_PREC_JAC_BLOCK) return cheby_run_block(ch_alphas, ch_betas, step_first,
-                                                             count, want_norm, norm);
+  if (!s.zb_all) return 0;
+  if (s.precond == TL_PREC_JAC_BLOCK)
+    return s.reflective ? 0 : cheby_run_block(ch_alphas, ch_betas, step_first, count, want_norm,
+                                              norm);
   const Geo G = geo_resident(2 * BLOCK, TL_CHEBY_PAIR_WAVES);
   const int nb = G.blocks < 0 ? -G.blocks : G.blocks;
   const int diag = s.precond == TL_PREC_JAC_DIAG;
   const bool nt = use_nontemporal();
-  auto k = nt ? k_cheby_v<true> : k_cheby_v<false>;
+  // reflective faces: one iteration per pass, reading u's halo cells
+  const bool refl = s.reflective;
+  auto k = nt ? (refl ? k_cheby_v<true, 2> : k_cheby_v<true, 0>)
+              : (refl ? k_cheby_v<false, 2> : k_cheby_v<false, 0>);
   using K2 = decltype(&k_cheby_pair<true, 0, 0>);
   static const K2 k2tab[2][2][2] = {
       {{&k_cheby_pair<false, 0, 0>, &k_cheby_pair<false, 0, 1>},
        {&k_cheby_pair<false, 1, 0>, &k_cheby_pair<false, 1, 1>}},
       {{&k_cheby_pair<true, 0, 0>, &k_cheby_pair<true, 0, 1>},
        {&k_cheby_pair<true, 1, 0>, &k_cheby_pair<true, 1, 1>}}};
-  const bool pairs = sweep_pairs();
+  const bool pairs = sweep_pairs() && !refl;
   for (int i = 0; i < count;) {
     const int st = step_first + i - 1;
     const int in = s.ucur, outb = 1 - s.ucur;

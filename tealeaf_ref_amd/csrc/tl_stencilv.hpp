@@ -112,8 +112,9 @@ struct ApplyA {};
 // same block are exchanged through LDS (one barrier per row) instead of being
 // loaded again; only the block's outer edges are loaded.  All waves of the
 // block must walk the same rows.
-// Outside the mesh the operand is mirrored (ZE = false) or zero (ZE = true).
-template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, bool ZE = false,
+// Outside the mesh the operand is mirrored (EDGE = 0), zero (EDGE = 1) or
+// read from the halo cells as they are in memory (EDGE = 2).
+template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, int EDGE = 0,
           typename LDV, typename LD1, typename EXTRA, typename BODY, typename CELL = ApplyA>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
@@ -133,8 +134,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
   const int cl = c <= nx ? c : (nx / V) * V;
   const int xr = tt.x0 + WAVE * V;  // first column right of this wave
   const bool edge_l = !LDSX || wave == 0, edge_r = !LDSX || wave == NW - 1;
-  const bool need_left = lane == 0 && tt.x0 > 0 && edge_l;
-  const bool need_right = lane == WAVE - 1 && xr < nx && edge_r;
+  const bool need_left = lane == 0 && (tt.x0 > 0 || EDGE == 2) && edge_l;
+  const bool need_right = lane == WAVE - 1 && (xr < nx || (EDGE == 2 && xr == nx)) && edge_r;
   const bool need_kxr = lane == WAVE - 1 && xr <= nx && edge_r;
   // Ky face between row y and the next row in walking direction
   auto face_ahead = [&](int y) -> long { return (long)(fwd ? y + 1 : y) * P + cl; };
@@ -143,8 +144,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
   // left and right of the wave; Kx right of the wave; the extra data
   VT vc = ldv_op(cl, ys);
   const int yb = ys - dir, ya = ys + dir;
-  VT vb = (yb >= 0 && yb < ny) ? ldv_op(cl, yb) : (ZE ? VT(0) : vc);
-  VT va = (ya >= 0 && ya < ny) ? ldv_op(cl, ya) : (ZE ? VT(0) : vc);
+  VT vb = (EDGE == 2 || (yb >= 0 && yb < ny)) ? ldv_op(cl, yb) : (EDGE == 1 ? VT(0) : vc);
+  VT va = (EDGE == 2 || (ya >= 0 && ya < ny)) ? ldv_op(cl, ya) : (EDGE == 1 ? VT(0) : vc);
   VT kb = ldv<NT, T, V>(Ky + (long)(fwd ? ys : ys + 1) * P + cl);
   VT ka = ldv<NT, T, V>(Ky + face_ahead(ys));
   VT kx = ldv<NT, T, V>(Kx + (long)ys * P + cl);
@@ -161,7 +162,7 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     E ex2 = ex;
     auto load_next = [&]() {
       const int yna = yn + dir;
-      va2 = (yna >= 0 && yna < ny) ? ldv_op(cl, yna) : (ZE ? VT(0) : va);
+      va2 = (EDGE == 2 || (yna >= 0 && yna < ny)) ? ldv_op(cl, yna) : (EDGE == 1 ? VT(0) : va);
       ka2 = ldv<NT, T, V>(Ky + face_ahead(yn));
       kx2 = ldv<NT, T, V>(Kx + (long)yn * P + cl);
       if (need_left) el2 = ld1_op(tt.x0 - 1, yn);
@@ -190,7 +191,7 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
     if (need_left) left = el;
     if (need_right) right = er;
     if (need_kxr) kxr = ek;
-    if (c == 0) left = ZE ? T(0) : vc[0];  // the left edge
+    if (c == 0 && EDGE != 2) left = EDGE == 1 ? T(0) : vc[0];  // the left edge
     const VT &vd = fwd ? vb : va, &vu = fwd ? va : vb;
     const VT &kyd = fwd ? kb : ka, &kyu = fwd ? ka : kb;
     VT w;
@@ -199,8 +200,8 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
       const int col = c + e;
       const T l = (e == 0) ? left : vc[e - 1];
       T r;
-      if (e + 1 < V) r = (col + 1 < nx) ? vc[e + 1] : (ZE ? T(0) : vc[e]);
-      else r = (col + 1 < nx) ? right : (ZE ? T(0) : vc[e]);
+      if (e + 1 < V) r = (col + 1 < nx || EDGE == 2) ? vc[e + 1] : (EDGE == 1 ? T(0) : vc[e]);
+      else r = (col + 1 < nx || EDGE == 2) ? right : (EDGE == 1 ? T(0) : vc[e]);
       const T kl = kx[e];
       const T kr = (e + 1 < V) ? kx[e + 1] : kxr;
       if constexpr (std::is_same<CELL, ApplyA>::value)

@@ -193,15 +193,21 @@ def test_jacobi_loop(pair_factory, reflective):
     p.cmp("u", tol=1e-10)
 
 
-def test_cheby_run(pair_factory):
-    p = pair_factory(64, 40, 1, seed=41)
+@pytest.mark.parametrize("precond,reflective", [(1, False), (1, True), (2, True), (3, True)])
+def test_cheby_run(pair_factory, precond, reflective):
+    """tl_cheby_run against the oracle's cheby_iterate.  With reflective faces
+    the fused pass reads u's halo cells as they are (the reference refreshes
+    only p's halo between Chebyshev iterations), one iteration per pass; with
+    reflective faces and the block preconditioner it runs per routine."""
+    print(f"cheby run precond={precond} reflective={reflective}", flush=True)
+    p = pair_factory(64, 40, 1, seed=41, precond=precond, reflective=reflective)
     _setup(p)
     theta = 20.0
     ch_a, ch_b, _ = O.calc_ch_coefs(1.0, 39.0, 30)
-    p.o.cheby_init(theta, 1)
+    p.o.cheby_init(theta, precond)
     L.call("tl_cheby_init", theta)
     for s in range(1, 21):
-        p.o.cheby_iterate(ch_a[s - 1], ch_b[s - 1], 1)
+        p.o.cheby_iterate(ch_a[s - 1], ch_b[s - 1], precond)
     rr_o = p.o.calc_2norm(1)
     a = np.ascontiguousarray(ch_a)
     b = np.ascontiguousarray(ch_b)
