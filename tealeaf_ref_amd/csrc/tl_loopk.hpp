@@ -20,10 +20,16 @@
 
 namespace tl {
 
+// stop: 0 while the loop runs; otherwise nonzero, and for a stop decided at
+// the head of the first kernel of iteration n it is stop_at(n), so that the
+// blocks of that same launch (which still apply the last p update) tell it
+// from a stop of an earlier launch.
 struct Ctl {
   int stop, reason, n_end, pad;
   double error, old_error, rro;
 };
+__host__ __device__ constexpr int stop_at(int n) { return n + 1; }
+constexpr int STOP_UPDATE = -1;  // a stop recorded by an update pass
 // tl_init allocates 64 device bytes for the control block and 256 pinned
 // host bytes for its copies (two batches in flight and the final read)
 static_assert(sizeof(Ctl) <= 64 && 3 * sizeof(Ctl) <= 256, "control block size");
@@ -215,7 +221,7 @@ __device__ inline Head loop_head(int n, int first, int tail, const LoopParams &l
       hbeta[n - 1] = h.beta;
       herr[n - 1] = err;
       if (h.stop) {
-        ctl->stop = 1;
+        ctl->stop = stop_at(n);
         ctl->reason = bad ? TL_LOOP_NONFINITE
                           : (conv ? TL_LOOP_CONVERGED : (sw ? TL_LOOP_SWITCH : TL_LOOP_MAXITERS));
         ctl->n_end = sw ? n : n - 1;
@@ -238,7 +244,7 @@ __device__ inline Head loop_head(int n, int first, int tail, const LoopParams &l
 __device__ inline bool nonfinite_stop(int n, double v, double v2, Ctl *ctl) {
   if (isfinite(v) && isfinite(v2)) return false;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctl->stop = 1;
+    ctl->stop = STOP_UPDATE;
     ctl->reason = TL_LOOP_NONFINITE;
     ctl->n_end = n;
     ctl->error = isfinite(v) ? v2 : v;
@@ -266,7 +272,12 @@ __global__ __launch_bounds__(BLOCK, TL_W_WAVES) void k_loop_w(Geo G, int n, int 
                                                    double *__restrict__ herr, int dir) {
   typedef typename dev::Vec<T, V>::type VT;
   __shared__ double lds[BLOCK / WAVE];
-  if (!first && ctl->stop) return;
+  // launches queued past the stop return at once; a block of the launch
+  // whose head stops the loop may already see that stop, and still runs
+  if (!first) {
+    const int st = ctl->stop;
+    if (st && st != stop_at(n)) return;
+  }
   const Head h =
       loop_head(n, first, tail, lp, rro_in, prev_part, nprev, ctl, hrro, hbeta, herr, lds);
   const T beta = (T)h.beta;

@@ -49,6 +49,27 @@ def test_tea_leaf_program(tmp_path):
     assert "This test is considered PASSED" in (tmp_path / "tea.out").read_text()
 
 
+def test_more_blocks_than_fit(tmp_path):
+    """The 1024^2 PPCG fixture with the loop kernels launched on 2048 tiles of
+    one row (TL_TILES=8192), more blocks than the GPU holds at once: blocks
+    that start after others have finished must still see the launch that
+    stops the CG presteps (and switches to PPCG) as their own, and apply its
+    last p update, which the PPCG iterations then use."""
+    import os
+    import subprocess
+    from tealeaf_ref_amd import teaout
+    name = "ppcg_large_1024"
+    fx = FX.load(name)
+    print(f"[e2e] {name} with TL_TILES=8192", flush=True)
+    (tmp_path / "tea.in").write_text(fx["deck"])
+    env = dict(os.environ, TL_TILES="8192")
+    subprocess.run([host.TEA_LEAF_BIN], cwd=tmp_path, check=True, timeout=120, env=env,
+                   stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    got = teaout.parse((tmp_path / "tea.out").read_text())
+    FX.compare_summaries(got["summaries"], fx["summaries"], RTOL)
+    FX.compare_iterations_exact(got["steps"], fx, name)
+
+
 @pytest.mark.timeout(400)
 @pytest.mark.parametrize("name", ["tea_bm_5", "tea_bm_6"])
 def test_benchmark_scale_deck(name, tmp_path):
