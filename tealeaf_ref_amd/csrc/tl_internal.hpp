@@ -164,9 +164,10 @@ namespace dev {
 // __shfl_down(v, 1) give it.  TL_DPP = 1 moves it with a DPP wave shift
 // (v_mov_b32 with wave_shr:1 / wave_shl:1, one VALU instruction per 32 bits:
 // lane i takes lane i - 1 / i + 1, the lane without a source keeps `old`),
-// TL_DPP = 0 through the LDS crossbar (ds_bpermute): the default, as the DPP
-// form made the CG w pass 8 % slower at 4096^2 (135 against 125 us) and
-// changed no other pass by more than 1 %.
+// TL_DPP = 0 through the LDS crossbar (ds_bpermute): the default; the DPP
+// form measured within 0.5 % of it on every pass at 4096^2 (builds whose
+// set-up kernels round alike, since the values in the fields change the
+// passes' times).
 #ifndef TL_DPP
 #define TL_DPP 0
 #endif
