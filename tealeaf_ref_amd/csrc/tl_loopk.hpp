@@ -122,9 +122,12 @@ constexpr bool diff_form() {
   return sizeof(T) == 4;
 }
 
-// Row prefetch of the stencil walks (see walkv) per kernel.
+// Row prefetch of the stencil walks (see walkv) per kernel.  The w pass
+// keeps two rows of loads in flight: 3 % faster than one at 4096^2 (110
+// registers, 4 waves per SIMD, as many as its 1024 tiles put on a CU);
+// three rows need more than 128 registers and measured no faster.
 #ifndef TL_PF_W
-#define TL_PF_W 1
+#define TL_PF_W 2
 #endif
 #ifndef TL_PF_SWEEP
 #define TL_PF_SWEEP 0

@@ -101,9 +101,11 @@ struct ApplyA {};
 // value (only for the two cells outside the wave); extra(row) -> the other
 // per-row loads of the body.  body(y, v, w, e) gets the operand vector, A
 // applied to it and extra(y).  DIFF selects the difference form of the
-// operator.  With PF every load of row y+1's step (operand row y+2, faces,
-// edge cells, extra) is issued before row y is computed, so one row of loads
-// is always in flight behind the arithmetic; the body itself must not load.
+// operator.  With PF = 1 every load of row y+1's step (operand row y+2,
+// faces, edge cells, extra) is issued before row y is computed, so one row
+// of loads is always in flight behind the arithmetic; with PF = d > 1 those
+// of the d steps ahead (d rows in flight, in more registers).  The body
+// itself must not load.
 // dir = -1 walks the rows from the top of the tile down (a pass that follows
 // one which ended at the top then reads the freshest rows first).
 // Lanes wholly outside [0, nx) run the loop too (their values feed the
@@ -114,7 +116,7 @@ struct ApplyA {};
 // block must walk the same rows.
 // Outside the mesh the operand is mirrored (EDGE = 0), zero (EDGE = 1) or
 // read from the halo cells as they are in memory (EDGE = 2).
-template <bool NT, typename T, int V, bool DIFF, bool PF, bool LDSX = false, int EDGE = 0,
+template <bool NT, typename T, int V, bool DIFF, int PF, bool LDSX = false, int EDGE = 0,
           typename LDV, typename LD1, typename EXTRA, typename BODY, typename CELL = ApplyA>
 __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restrict__ Kx,
                              const T *__restrict__ Ky, T rx, T ry, LDV ldv_op, LD1 ld1_op,
@@ -153,24 +155,37 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
   T er = need_right ? ld1_op(xr, ys) : T(0);
   T ek = need_kxr ? Kx[(long)ys * P + xr] : T(0);
   E ex = extra(ys);
+  // the loads that iteration j takes over when it becomes current: the
+  // operand row beyond y_j (`in` false where it lies outside the mesh and
+  // is resolved on taking over, from rows already loaded), the Ky face ahead
+  // of y_j, Kx, the edge cells and the extra data of y_j
+  struct Stage {
+    VT v, ka, kx;
+    T el, er, ek;
+    E ex;
+    bool in;
+  };
+  auto load_stage = [&](int j, Stage &o) {
+    const int yj = ys + j * dir, yo = yj + dir;
+    o.in = EDGE == 2 || (yo >= 0 && yo < ny);
+    o.v = o.in ? ldv_op(cl, yo) : VT(0);
+    o.ka = ldv<NT, T, V>(Ky + face_ahead(yj));
+    o.kx = ldv<NT, T, V>(Kx + (long)yj * P + cl);
+    o.el = need_left ? ld1_op(tt.x0 - 1, yj) : T(0);
+    o.er = need_right ? ld1_op(xr, yj) : T(0);
+    o.ek = need_kxr ? Kx[(long)yj * P + xr] : T(0);
+    o.ex = extra(yj);
+  };
+  // q[i]: iteration k + 1 + i, the last one loaded during iteration k
+  constexpr int D = PF > 0 ? PF : 1;
+  Stage q[D];
+#pragma unroll
+  for (int i = 0; i + 1 < D; ++i)
+    if (1 + i < count) load_stage(1 + i, q[i]);
   for (int k = 0; k < count; ++k) {
     const int y = ys + k * dir;
-    const int yn = y + dir;
     const bool more = k + 1 < count;
-    VT va2 = va, ka2 = ka, kx2 = kx;
-    T el2 = el, er2 = er, ek2 = ek;
-    E ex2 = ex;
-    auto load_next = [&]() {
-      const int yna = yn + dir;
-      va2 = (EDGE == 2 || (yna >= 0 && yna < ny)) ? ldv_op(cl, yna) : (EDGE == 1 ? VT(0) : va);
-      ka2 = ldv<NT, T, V>(Ky + face_ahead(yn));
-      kx2 = ldv<NT, T, V>(Kx + (long)yn * P + cl);
-      if (need_left) el2 = ld1_op(tt.x0 - 1, yn);
-      if (need_right) er2 = ld1_op(xr, yn);
-      if (need_kxr) ek2 = Kx[(long)yn * P + xr];
-      ex2 = extra(yn);
-    };
-    if (PF && more) load_next();
+    if (PF > 0 && k + PF < count) load_stage(k + PF, q[D - 1]);
     T left = from_lower_lane(vc[V - 1]);
     T right = from_upper_lane(vc[0]);
     T kxr = from_upper_lane(kx[0]);
@@ -211,17 +226,23 @@ __device__ inline void walkv(const Geo &g, const TileV<V> &tt, const T *__restri
         w[e] = cell(ex, e, vc[e], l, r, vd[e], vu[e], kl, kr, kyd[e], kyu[e]);
     }
     body(y, vc, w, ex);
-    if (!PF && more) load_next();
+    if (!more) break;
+    if (PF == 0) load_stage(k + 1, q[0]);
+    // iteration k + 1 takes over: outside the mesh the operand row is the
+    // mirror of the row at the edge (EDGE = 0) or zero (EDGE = 1)
+    const VT nv = q[0].in ? q[0].v : (EDGE == 1 ? VT(0) : va);
     vb = vc;
     vc = va;
-    va = va2;
+    va = nv;
     kb = ka;
-    ka = ka2;
-    kx = kx2;
-    el = el2;
-    er = er2;
-    ek = ek2;
-    ex = ex2;
+    ka = q[0].ka;
+    kx = q[0].kx;
+    el = q[0].el;
+    er = q[0].er;
+    ek = q[0].ek;
+    ex = q[0].ex;
+#pragma unroll
+    for (int i = 0; i + 1 < D; ++i) q[i] = q[i + 1];
   }
 }
 
