@@ -232,7 +232,7 @@ extern "C" int tl_init(int nx, int ny, int halo, int device) {
   TL_HIP(hipMemsetAsync(s.partials, 0, sizeof(double) * MAX_GRID * NPART, s.stream));
   TL_HIP(hipMalloc((void **)&s.dscal, sizeof(double) * 64));
   TL_HIP(hipHostMalloc((void **)&s.hscal, sizeof(double) * 64, hipHostMallocDefault));
-  // the fused loops' control block, its host copies and the driver's events
+  // the fused loops' control block, its host copies and the loop driver's batch events
   // (set up here so that a loop call allocates nothing but its history)
   TL_HIP(hipMalloc(&s.ctl, 64));
   TL_HIP(hipMemsetAsync(s.ctl, 0, 64, s.stream));
