@@ -131,22 +131,7 @@ static int copy_slot(int slot, double *host, int host_halo, bool up) {
   TL_REQUIRE_INIT();
   int rc = check_slot(slot);
   if (rc) return rc;
-  State &s = S();
-  if (!host) return fail(TL_ERR_ARG, "null host pointer");
-  if (host_halo < 0) return fail(TL_ERR_ARG, "negative host halo");
-  const int hh = host_halo < s.h ? host_halo : s.h;
-  const long hw = (long)s.nx + 2L * host_halo;
-  const long cols = (long)s.nx + 2L * hh, rows = (long)s.ny + 2L * hh;
-  double *hp = host + (long)(host_halo - hh) + (long)(host_halo - hh) * hw;
-  double *dp = snap11(slot) - hh - (long)hh * s.pitch;
-  if (up)
-    TL_HIP(hipMemcpy2DAsync(dp, s.pitch * sizeof(double), hp, hw * sizeof(double),
-                            cols * sizeof(double), rows, hipMemcpyHostToDevice, s.stream));
-  else
-    TL_HIP(hipMemcpy2DAsync(hp, hw * sizeof(double), dp, s.pitch * sizeof(double),
-                            cols * sizeof(double), rows, hipMemcpyDeviceToHost, s.stream));
-  TL_HIP(hipStreamSynchronize(s.stream));
-  return TL_OK;
+  return copy_host(snap11(slot), host, host_halo, up);
 }
 
 extern "C" int tl_snapshot_upload(int slot, const double *host, int host_halo) {

@@ -80,6 +80,10 @@ struct State {
   size_t field32 = 0;          // floats per field
   size_t f32_bytes = 0;        // the whole FP32 allocation
 
+  // device staging buffer of host copies (allocated on first use)
+  double *stage = nullptr;
+  size_t stage_len = 0;       // doubles
+
   // snapshot slots (checkpoints and work arrays of the adjoint), each laid
   // out exactly like a field allocation
   std::vector<double *> snaps;
@@ -140,6 +144,11 @@ int finish_sums(const int *slots, int nslots, int n, double *out);
 
 // field of a TL_FIELD_* id (the current buffer of ping-ponged fields)
 Field *field_by_id(int id);
+
+// synchronous copy of the cells inside both halos between a caller's host
+// array (halo host_halo) and the pitched device array whose cell (1,1) is
+// at p11 (tl_runtime.hip)
+int copy_host(double *p11, double *host, int host_halo, bool up);
 
 // checks of a caller's ny x nx device array (row stride ld) and a
 // synchronous copy of the interior cells between it and p11 (tl_runtime.hip)

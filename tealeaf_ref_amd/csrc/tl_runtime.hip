@@ -190,6 +190,7 @@ extern "C" int tl_finalize(void) {
   for (hipEvent_t e : s.loop_ev)
     if (e) (void)hipEventDestroy(e);
   if (s.f32) (void)hipFree(s.f32);
+  if (s.stage) (void)hipFree(s.stage);
   for (double *q : s.snaps) (void)hipFree(q);
   for (auto &ev : s.ev_pending) {
     (void)hipEventDestroy(ev.a);
@@ -302,27 +303,68 @@ Field *field_by_id(int id) {
 }
 }  // namespace tl
 
-// Copy the rectangle of cells inside both halos.
-static int copy_field(int id, double *host, int host_halo, bool up) {
-  TL_REQUIRE_INIT();
+namespace tl {
+// The device staging buffer, grown to n doubles; null if that fails.
+static double *staging(size_t n) {
   State &s = S();
-  Field *f = field_by_id(id);
-  if (!f) return fail(TL_ERR_ARG, "unknown field id " + std::to_string(id));
+  if (s.stage_len < n) {
+    if (s.stage) (void)hipFree(s.stage);
+    s.stage = nullptr;
+    s.stage_len = 0;
+    if (hipMalloc((void **)&s.stage, n * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();  // clear it: the caller copies without staging
+      s.stage = nullptr;
+      return nullptr;
+    }
+    s.stage_len = n;
+  }
+  return s.stage;
+}
+
+// A 2D copy straight between pageable host memory and a pitched device array
+// moves about 3 GB/s on MI355X; one contiguous copy over PCIe to or from the
+// staging buffer plus a 2D copy on the device is several times faster.  A
+// download stages only when the host rows are contiguous (the copy back then
+// writes no host cell outside the rectangle).
+int copy_host(double *p11, double *host, int host_halo, bool up) {
+  State &s = S();
   if (!host) return fail(TL_ERR_ARG, "null host pointer");
   if (host_halo < 0) return fail(TL_ERR_ARG, "negative host halo");
   const int hh = host_halo < s.h ? host_halo : s.h;
   const long hw = (long)s.nx + 2L * host_halo;           // host row length
   const long cols = (long)s.nx + 2L * hh, rows = (long)s.ny + 2L * hh;
   double *hp = host + (long)(host_halo - hh) + (long)(host_halo - hh) * hw;
-  double *dp = f->p11 - hh - (long)hh * s.pitch;
-  if (up)
-    TL_HIP(hipMemcpy2DAsync(dp, s.pitch * sizeof(double), hp, hw * sizeof(double),
-                            cols * sizeof(double), rows, hipMemcpyHostToDevice, s.stream));
-  else
+  double *dp = p11 - hh - (long)hh * s.pitch;
+  const size_t span = (size_t)((rows - 1) * hw + cols);  // host doubles from hp to the last cell
+  double *st = (up || hw == cols) ? staging(span) : nullptr;
+  if (up) {
+    if (st) {
+      TL_HIP(hipMemcpyAsync(st, hp, span * sizeof(double), hipMemcpyHostToDevice, s.stream));
+      TL_HIP(hipMemcpy2DAsync(dp, s.pitch * sizeof(double), st, hw * sizeof(double),
+                              cols * sizeof(double), rows, hipMemcpyDeviceToDevice, s.stream));
+    } else {
+      TL_HIP(hipMemcpy2DAsync(dp, s.pitch * sizeof(double), hp, hw * sizeof(double),
+                              cols * sizeof(double), rows, hipMemcpyHostToDevice, s.stream));
+    }
+  } else if (st) {
+    TL_HIP(hipMemcpy2DAsync(st, cols * sizeof(double), dp, s.pitch * sizeof(double),
+                            cols * sizeof(double), rows, hipMemcpyDeviceToDevice, s.stream));
+    TL_HIP(hipMemcpyAsync(hp, st, span * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  } else {
     TL_HIP(hipMemcpy2DAsync(hp, hw * sizeof(double), dp, s.pitch * sizeof(double),
                             cols * sizeof(double), rows, hipMemcpyDeviceToHost, s.stream));
+  }
   TL_HIP(hipStreamSynchronize(s.stream));
   return TL_OK;
+}
+}  // namespace tl
+
+// Copy the rectangle of cells inside both halos.
+static int copy_field(int id, double *host, int host_halo, bool up) {
+  TL_REQUIRE_INIT();
+  Field *f = field_by_id(id);
+  if (!f) return fail(TL_ERR_ARG, "unknown field id " + std::to_string(id));
+  return copy_host(f->p11, host, host_halo, up);
 }
 
 extern "C" int tl_upload(int field_id, const double *host, int host_halo) {
@@ -406,6 +448,7 @@ extern "C" int tl_device_bytes(double *bytes) {
   b += (double)s.f32_bytes;
   b += sizeof(double) * (double)MAX_GRID * NPART;
   if (s.hist) b += sizeof(double) * 4.0 * (double)s.hist_len;
+  b += sizeof(double) * (double)s.stage_len;
   *bytes = b;
   return TL_OK;
 }
