@@ -86,7 +86,9 @@ int tl_set_ppcg_active(int active);
 
 /* Copy a host array with `host_halo` halo cells to/from a device field.
  * Only the cells inside both halos are copied.  volume uses host_halo = 2
- * (kernels/field_summary_kernel.f90:36); cp and bfp use host_halo = 0. */
+ * (kernels/field_summary_kernel.f90:36); cp and bfp use host_halo = 0.
+ * The copy goes through a device staging buffer of its size, which the
+ * library keeps until tl_finalize (the snapshot copies share it). */
 int tl_upload(int field_id, const double *host, int host_halo);
 int tl_download(int field_id, double *host, int host_halo);
 
