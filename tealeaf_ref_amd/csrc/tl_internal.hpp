@@ -8,6 +8,7 @@
 // (1,1) and index it with 0-based (x, y) = (j-1, k-1).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -98,6 +99,8 @@ struct State {
   long prof_count[8] = {0};
   double prof_ms[8] = {0};
   long prof_idle[8] = {0};  // timed launches that returned at once
+  hipEvent_t arm_a = nullptr, arm_b = nullptr;  // events of a prof_arm region
+  bool arm_fired = false;                       // ... taken by its launch
 
   std::string err;
 };
@@ -134,9 +137,26 @@ struct Grid {
 Grid grid_for(int nx, int ny, int cols_per_tile = BLOCK, int target_tiles = MAX_GRID,
               int max_rows = 32);
 
-// profiling helpers (no-ops unless enabled)
+// profiling helpers (no-ops unless enabled).  prof_begin / prof_end time a
+// region between two marker events on the stream; prof_arm / prof_end time
+// the one kernel of the region launched through klaunch(), whose dispatch
+// records its own start and end (no marker packets around it).
 void prof_begin(int cls, hipEvent_t *a);
+void prof_arm(int cls, hipEvent_t *a);
 void prof_end(int cls, hipEvent_t a);
+
+// Launch on the library's stream; inside a prof_arm region the launch takes
+// the region's events.
+template <typename... F, typename... A>
+inline void klaunch(void (*k)(F...), dim3 grid, dim3 block, A... args) {
+  State &s = S();
+  if (s.prof && s.arm_a && !s.arm_fired) {
+    s.arm_fired = true;
+    hipExtLaunchKernelGGL(k, grid, block, 0, s.stream, s.arm_a, s.arm_b, 0, args...);
+  } else {
+    hipLaunchKernelGGL(k, grid, block, 0, s.stream, args...);
+  }
+}
 
 // finish a reduction: sum `n` partials of `slot` (deterministically) into
 // dscal[idx]; returns after copying it to the host (synchronises)

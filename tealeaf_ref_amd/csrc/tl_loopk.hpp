@@ -986,7 +986,6 @@ template <typename T, int V>
 int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas, int inner,
                  T theta_r, int want_rrn, const Ctl *ctl, const LoopMem &m, int &sd, T *&rt_cur,
                  int dir) {
-  State &s = S();
   const Geo G = b.G;
   const int nb = abs_blocks(G);
   const bool nt = use_nontemporal();
@@ -1004,12 +1003,12 @@ int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas
            {&k_pp_sweep_blk<true, T, V, 1, 0>, &k_pp_sweep_blk<true, T, V, 1, 1>}}};
       for (int i = 1; i <= inner; ++i) {
         hipEvent_t ev;
-        prof_begin(2, &ev);
-        hipLaunchKernelGGL(kb[nt ? 1 : 0][i == 1 ? 1 : 0][i == inner ? 1 : 0], dim3(nb),
-                           dim3(BLOCK), 0, s.stream, G, want_rrn, theta_r, (T)alphas[i - 1],
-                           (T)betas[i - 1], b.z, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r,
-                           b.r_store, b.z, b.Di, b.cp, b.bfp, b.Kx, b.Ky, b.rx, b.ry,
-                           m.part(P3), m.part(P2), ctl);
+        prof_arm(2, &ev);
+        klaunch(kb[nt ? 1 : 0][i == 1 ? 1 : 0][i == inner ? 1 : 0], dim3(nb),
+                dim3(BLOCK), G, want_rrn, theta_r, (T)alphas[i - 1],
+                (T)betas[i - 1], b.z, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r,
+                b.r_store, b.z, b.Di, b.cp, b.bfp, b.Kx, b.Ky, b.rx, b.ry,
+                m.part(P3), m.part(P2), ctl);
         prof_end(2, ev);
         sd = 1 - sd;
       }
@@ -1022,22 +1021,22 @@ int queue_sweeps(const LoopBufs<T> &b, const double *alphas, const double *betas
   const bool pairs = sweep_pairs();
   for (int i = 1; i <= inner;) {
     hipEvent_t ev;
-    prof_begin(2, &ev);
+    prof_arm(2, &ev);
     if (pairs && i + 1 <= inner) {
       T *rt_next = rt_cur == b.rtemp ? b.w : b.rtemp;
       auto k2 = nt ? sweep2_kernel<true, T, V>(i == 1, i + 1 == inner, b.diag)
                    : sweep2_kernel<false, T, V>(i == 1, i + 1 == inner, b.diag);
-      hipLaunchKernelGGL(k2, dim3(nb), dim3(BLOCK), 0, s.stream, G, want_rrn, theta_r,
-                         (T)alphas[i - 1], (T)betas[i - 1], (T)alphas[i], (T)betas[i], src,
-                         b.sd[sd], b.sd[1 - sd], rt_cur, rt_next, b.utemp, b.r, b.r_store, b.z,
-                         b.Mi, b.Kx, b.Ky, b.rx, b.ry, m.part(P3), m.part(P2), ctl, dir);
+      klaunch(k2, dim3(nb), dim3(BLOCK), G, want_rrn, theta_r,
+              (T)alphas[i - 1], (T)betas[i - 1], (T)alphas[i], (T)betas[i], src,
+              b.sd[sd], b.sd[1 - sd], rt_cur, rt_next, b.utemp, b.r, b.r_store, b.z,
+              b.Mi, b.Kx, b.Ky, b.rx, b.ry, m.part(P3), m.part(P2), ctl, dir);
       rt_cur = rt_next;
       i += 2;
     } else {
-      hipLaunchKernelGGL(k1, dim3(nb), dim3(BLOCK), 0, s.stream, G, i == 1 ? 1 : 0,
-                         i == inner ? 1 : 0, want_rrn, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
-                         b.diag, src, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r, b.r_store,
-                         b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, m.part(P3), m.part(P2), ctl, dir);
+      klaunch(k1, dim3(nb), dim3(BLOCK), G, i == 1 ? 1 : 0,
+              i == inner ? 1 : 0, want_rrn, theta_r, (T)alphas[i - 1], (T)betas[i - 1],
+              b.diag, src, b.sd[sd], b.sd[1 - sd], rt_cur, b.utemp, b.r, b.r_store,
+              b.z, b.Mi, b.Kx, b.Ky, b.rx, b.ry, m.part(P3), m.part(P2), ctl, dir);
       if (pairs || (i & 1) == 0) dir = -dir;
       i += 1;
       prof_end(2, ev);
@@ -1079,20 +1078,20 @@ int run_cg_loop(const LoopBufs<T> &b, tl_loop_state *st, double *cg_alphas, doub
     const int cin = c0 ^ ((n - st->n_first) & 1);
     const T *pin = (first && pin_first) ? pin_first : b.p[cin];
     hipEvent_t ev;
-    prof_begin(0, &ev);
-    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
-                       st->rro, b.cg_src, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
-                       m.part(P2), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, 1);
+    prof_arm(0, &ev);
+    klaunch(kw, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0, 0, lp,
+            st->rro, b.cg_src, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
+            m.part(P2), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, 1);
     prof_end(0, ev);
-    prof_begin(1, &ev);
+    prof_arm(1, &ev);
     if (b.block)
-      hipLaunchKernelGGL(kub, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
-                         b.w, b.z, b.Di, b.Ky, b.cp, b.bfp, b.ry, m.part(P1), nb, m.part(P2),
-                         m.ctl, m.hrro, m.halpha, 1);
+      klaunch(kub, dim3(nb), dim3(BLOCK), G, n, b.u, b.p[1 - cin], b.r,
+              b.w, b.z, b.Di, b.Ky, b.cp, b.bfp, b.ry, m.part(P1), nb, m.part(P2),
+              m.ctl, m.hrro, m.halpha, 1);
     else
-      hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, b.u, b.p[1 - cin], b.r,
-                         b.w, b.z, b.Mi, b.diag, m.part(P1), nb, m.part(P2), m.ctl, m.hrro,
-                         m.halpha, 1);
+      klaunch(kur, dim3(nb), dim3(BLOCK), G, n, b.u, b.p[1 - cin], b.r,
+              b.w, b.z, b.Mi, b.diag, m.part(P1), nb, m.part(P2), m.ctl, m.hrro,
+              m.halpha, 1);
     prof_end(1, ev);
     TL_HIP(hipGetLastError());
     return TL_OK;
@@ -1155,27 +1154,27 @@ int run_ppcg_loop(const LoopBufs<T> &b, tl_loop_state *st, const double *alphas,
     const T *pin = (first && pin_first) ? pin_first : b.p[cin];
     const int d = ((passes & 1) && ((n - st->n_first) & 1)) ? -1 : 1;
     hipEvent_t ev;
-    prof_begin(5, &ev);
-    hipLaunchKernelGGL(kw, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0, 0, lp,
-                       rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
-                       m.part(P3), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, d);
+    prof_arm(5, &ev);
+    klaunch(kw, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0, 0, lp,
+            rro_first, b.z, pin, b.p[1 - cin], b.Kx, b.Ky, b.w, b.rx, b.ry,
+            m.part(P3), nb, m.part(P1), m.ctl, m.hrro, m.hbeta, m.herr, d);
     prof_end(5, ev);
-    prof_begin(6, &ev);
+    prof_arm(6, &ev);
     bool block_ur = false;
     if constexpr (sizeof(T) == 8) {
       if (b.block) {
         auto kub = nt ? k_pp_ur_blk<true, T, V> : k_pp_ur_blk<false, T, V>;
-        hipLaunchKernelGGL(kub, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
-                           rro_first, b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Di, b.Ky,
-                           b.cp, b.bfp, b.ry, m.part(P1), m.part(P2), nb, m.ctl, m.hrro,
-                           m.halpha, d > 0 ? 1 : 0);
+        klaunch(kub, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0,
+                rro_first, b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Di, b.Ky,
+                b.cp, b.bfp, b.ry, m.part(P1), m.part(P2), nb, m.ctl, m.hrro,
+                m.halpha, d > 0 ? 1 : 0);
         block_ur = true;
       }
     }
     if (!block_ur)
-      hipLaunchKernelGGL(kur, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
-                         rro_first, b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag,
-                         m.part(P1), m.part(P2), nb, m.ctl, m.hrro, m.halpha, d > 0 ? 1 : 0);
+      klaunch(kur, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0,
+              rro_first, b.u, b.p[1 - cin], b.r, b.r_store, b.w, b.z, b.Mi, b.diag,
+              m.part(P1), m.part(P2), nb, m.ctl, m.hrro, m.halpha, d > 0 ? 1 : 0);
     prof_end(6, ev);
     T *rt_cur = b.rtemp;
     return queue_sweeps<T, V>(b, alphas, betas, inner, theta_r, 1, m.ctl, m, sd, rt_cur, d);

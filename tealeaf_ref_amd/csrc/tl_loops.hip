@@ -571,11 +571,11 @@ int cheby_run_block(const double *ch_alphas, const double *ch_betas, int step_fi
     auto k = nt ? (last ? k_cheby_blk<true, 1> : k_cheby_blk<true, 0>)
                 : (last ? k_cheby_blk<false, 1> : k_cheby_blk<false, 0>);
     hipEvent_t ev;
-    prof_begin(3, &ev);
-    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
-                       s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
-                       s.Di.p11, s.cp.p11, s.bfp.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st],
-                       part(P0));
+    prof_arm(3, &ev);
+    klaunch(k, dim3(nb), dim3(BLOCK), G, s.u[in].p11, s.u[outb].p11,
+            s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
+            s.Di.p11, s.cp.p11, s.bfp.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st],
+            part(P0));
     prof_end(3, ev);
     if (hipGetLastError() != hipSuccess) return -fail(TL_ERR_HIP, "Chebyshev pass launch");
     s.ucur = outb;
@@ -617,23 +617,23 @@ int cheby_run_fused(const double *ch_alphas, const double *ch_betas, int step_fi
     const int st = step_first + i - 1;
     const int in = s.ucur, outb = 1 - s.ucur;
     hipEvent_t ev;
-    prof_begin(3, &ev);
+    prof_arm(3, &ev);
     if (pairs && i + 1 < count) {
       // two iterations per pass; p moves to the other buffer
       const int last = i + 2 == count;
-      hipLaunchKernelGGL(k2tab[nt ? 1 : 0][diag][last], dim3(nb), dim3(BLOCK), 0, s.stream, G,
-                         s.u[in].p11, s.u[outb].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
-                         s.p[s.pcur].p11, s.p[1 - s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
-                         s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], ch_alphas[st + 1],
-                         ch_betas[st + 1], part(P0), s.walk_dir);
+      klaunch(k2tab[nt ? 1 : 0][diag][last], dim3(nb), dim3(BLOCK), G,
+              s.u[in].p11, s.u[outb].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
+              s.p[s.pcur].p11, s.p[1 - s.pcur].p11, s.w.p11, s.r.p11, s.z.p11,
+              s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], ch_alphas[st + 1],
+              ch_betas[st + 1], part(P0), s.walk_dir);
       s.pcur = 1 - s.pcur;
       i += 2;
     } else {
       const int last = i + 1 == count;
-      hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.u[in].p11, s.u[outb].p11,
-                         s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11,
-                         s.z.p11, s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], diag, last,
-                         part(P0), s.walk_dir);
+      klaunch(k, dim3(nb), dim3(BLOCK), G, s.u[in].p11, s.u[outb].p11,
+              s.u0.p11, s.Kx.p11, s.Ky.p11, s.p[s.pcur].p11, s.w.p11, s.r.p11,
+              s.z.p11, s.Mi.p11, s.rx, s.ry, ch_alphas[st], ch_betas[st], diag, last,
+              part(P0), s.walk_dir);
       i += 1;
     }
     s.walk_dir = -s.walk_dir;  // the next pass starts on the rows written last
@@ -1022,23 +1022,22 @@ int jacobi_loop_pairs(tl_loop_state *st) {
     const int j = off / 2, in = c0 ^ (j & 1);
     const double *pa = first ? nullptr : slot_a(j - 1), *pb = first ? nullptr : slot_b(j - 1);
     const int dir = (j & 1) ? -1 : 1;
-    hipEvent_t ev;
-    prof_begin(4, &ev);
-    if (!is_single(j)) {
-      hipLaunchKernelGGL(kp, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
-                         st->eps_abs, s.u[in].p11, mid, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
-                         s.Ky.p11, s.rx, s.ry, pa, pb, nb, slot_a(j), slot_b(j), ctl, herr, dir);
-    } else if (first) {
-      hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 1, st->eps_abs,
-                         s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx,
-                         s.ry, slot_b(j), nb, slot_b(j), ctl, herr, dir);
-    } else {
-      // exit test of the previous pair, then one iteration
+    // a single iteration after a pair: the exit test of the pair first
+    if (is_single(j) && !first)
       hipLaunchKernelGGL(k_jl_pair_test, dim3(1), dim3(BLOCK), 0, s.stream, n, st->eps_abs, pa,
                          pb, nb, ctl, herr, 0);
-      hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, 0, st->eps_abs,
-                         s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx,
-                         s.ry, pb, nb, slot_b(j), ctl, herr, dir);
+    hipEvent_t ev;
+    prof_arm(4, &ev);
+    if (!is_single(j)) {
+      klaunch(kp, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0, st->eps_abs, s.u[in].p11, mid,
+              s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, pa, pb, nb, slot_a(j),
+              slot_b(j), ctl, herr, dir);
+    } else if (first) {
+      klaunch(kv, dim3(nb), dim3(BLOCK), G, n, 1, st->eps_abs, s.u[in].p11, s.u[1 - in].p11,
+              s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, slot_b(j), nb, slot_b(j), ctl, herr, dir);
+    } else {
+      klaunch(kv, dim3(nb), dim3(BLOCK), G, n, 0, st->eps_abs, s.u[in].p11, s.u[1 - in].p11,
+              s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, pb, nb, slot_b(j), ctl, herr, dir);
     }
     prof_end(4, ev);
     TL_HIP(hipGetLastError());
@@ -1101,19 +1100,19 @@ extern "C" int tl_jacobi_loop(tl_loop_state *st) {
   auto launch = [&](int n, bool first) -> int {
     const int in = u0c ^ ((n - st->n_first) & 1);
     hipEvent_t ev;
-    prof_begin(4, &ev);
+    prof_arm(4, &ev);
     if (vec) {
       // consecutive sweeps walk their rows in opposite directions
       const int dir = ((n - st->n_first) & 1) ? -1 : 1;
-      hipLaunchKernelGGL(kv, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
-                         st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
-                         s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb, part(n & 1 ? P4 : P5), ctl,
-                         herr, dir);
+      klaunch(kv, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0,
+              st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11, s.Ky.p11,
+              s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb, part(n & 1 ? P4 : P5), ctl,
+              herr, dir);
     } else {
-      hipLaunchKernelGGL(k_jl, dim3(nb), dim3(BLOCK), 0, s.stream, G, n, first ? 1 : 0,
-                         st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
-                         s.Ky.p11, s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb,
-                         part(n & 1 ? P4 : P5), ctl, herr);
+      klaunch(k_jl, dim3(nb), dim3(BLOCK), G, n, first ? 1 : 0,
+              st->eps_abs, s.u[in].p11, s.u[1 - in].p11, s.u0.p11, s.Kx.p11,
+              s.Ky.p11, s.rx, s.ry, part((n - 1) & 1 ? P4 : P5), nb,
+              part(n & 1 ? P4 : P5), ctl, herr);
     }
     prof_end(4, ev);
     TL_HIP(hipGetLastError());

@@ -284,15 +284,15 @@ int refine(bool with_d, double *rr64, double *rr32) {
   const double *uin = s.u[s.ucur].p11;
   double *uout = s.u[1 - s.ucur].p11;
   hipEvent_t ev;
-  prof_begin(7, &ev);
+  prof_arm(7, &ev);
   if (with_d) {
     auto k = nt ? k_mx_refine<true, true> : k_mx_refine<false, true>;
-    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.pitch32, uin, f32(D), uout,
-                       s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, f32(R), part(P4), part(P5));
+    klaunch(k, dim3(nb), dim3(BLOCK), G, s.pitch32, uin, f32(D), uout,
+            s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, f32(R), part(P4), part(P5));
   } else {
     auto k = nt ? k_mx_refine<true, false> : k_mx_refine<false, false>;
-    hipLaunchKernelGGL(k, dim3(nb), dim3(BLOCK), 0, s.stream, G, s.pitch32, uin, f32(D), uout,
-                       s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, f32(R), part(P4), part(P5));
+    klaunch(k, dim3(nb), dim3(BLOCK), G, s.pitch32, uin, f32(D), uout,
+            s.u0.p11, s.Kx.p11, s.Ky.p11, s.rx, s.ry, f32(R), part(P4), part(P5));
   }
   prof_end(7, ev);
   TL_HIP(hipGetLastError());

@@ -98,9 +98,47 @@ void prof_begin(int cls, hipEvent_t *a) {
   (void)hipEventRecord(*a, s.stream);
 }
 
+void prof_arm(int cls, hipEvent_t *a) {
+  State &s = S();
+  *a = nullptr;
+  if (!s.prof || cls < 0 || cls >= 8) return;
+  if ((s.prof_seq[cls]++ % s.prof_period) != 0) return;
+  hipEvent_t e[2] = {nullptr, nullptr};
+  for (hipEvent_t &x : e) {
+    if (s.ev_pool.empty()) {
+      if (hipEventCreate(&x) != hipSuccess) {
+        for (hipEvent_t y : e)
+          if (y) s.ev_pool.push_back(y);
+        return;
+      }
+    } else {
+      x = s.ev_pool.back();
+      s.ev_pool.pop_back();
+    }
+  }
+  s.arm_a = e[0];
+  s.arm_b = e[1];
+  s.arm_fired = false;
+  *a = e[0];
+}
+
 void prof_end(int cls, hipEvent_t a) {
   State &s = S();
   if (!s.prof || !a) return;
+  if (a == s.arm_a) {  // a prof_arm region: its launch recorded both events
+    const hipEvent_t b = s.arm_b;
+    const bool fired = s.arm_fired;
+    s.arm_a = s.arm_b = nullptr;
+    s.arm_fired = false;
+    if (!fired) {  // nothing was launched
+      s.ev_pool.push_back(a);
+      s.ev_pool.push_back(b);
+      return;
+    }
+    s.ev_pending.push_back({a, b, cls});
+    if (s.ev_pending.size() >= 4096) prof_resolve();
+    return;
+  }
   hipEvent_t b;
   if (s.ev_pool.empty()) {
     if (hipEventCreate(&b) != hipSuccess) return;
