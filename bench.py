@@ -33,8 +33,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HIP events bracket every EVENT_PERIOD-th launch of each kernel class in the
-# timed steps: events around every launch slow a CG step by ~6 %
+# every EVENT_PERIOD-th launch of each kernel class in the timed steps records
+# its own start and end in HIP events (timing every launch slows a CG step
+# by a few per cent)
 EVENT_PERIOD = 16
 CG_W_BYTES_PER_CELL = 48.0      # fused calc_w: r, p, Kx, Ky in; p, w out
 CG_UR_BYTES_PER_CELL = 48.0     # calc_ur: u, p, r, w in; u, r out
