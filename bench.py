@@ -902,12 +902,12 @@ def main():
                                        save=args.adjoint_save, compare=args.adjoint_compare)
     if args.adjoint_large_cells and args.adjoint_large_steps:
         # the mixed-precision + adjoint case at full size, bounded: FP64 and
-        # mixed CG forward and backward over a few steps, checkpoint every
-        # step (the 50-step case runs as two invocations, --adjoint-save then
-        # --adjoint-compare)
+        # mixed PPCG (its fastest form) forward and backward over a few
+        # steps, checkpoint every step (the 50-step case runs as two
+        # invocations, --adjoint-save then --adjoint-compare)
         out["adjoint_large"] = adjoint_bench(
             args.adjoint_large_cells, args.adjoint_large_steps, 1, 0, args.adjoint_h, "both",
-            args.adjoint_mixed_eps, solver="cg")
+            args.adjoint_mixed_eps, solver="ppcg")
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cells, args.solver, mean_iters, args.cpu_cap,
