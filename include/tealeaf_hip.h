@@ -221,9 +221,10 @@ typedef struct {
                               * does not check (it iterates to max_iters); a
                               * zero density gives such values through Kx/Ky
                               * (kernels/tea_leaf_common_kernel.f90:97-98). */
-#define TL_LOOP_FLOOR 5      /* tl_mixed_solve only: a refinement failed to
-                              * halve the true residual, which has reached
-                              * its rounding floor above rel_tol             */
+#define TL_LOOP_FLOOR 5      /* tl_mixed_solve only: the true residual has
+                              * reached its rounding floor above rel_tol (a
+                              * refinement whose inner solve gained 4x, or
+                              * two in a row, failed to halve it)            */
 
 /* CG branch (tea_solve.f90:341-369).  cg_alphas/cg_betas are host arrays
  * indexed by n (entry n written for every executed iteration), as
@@ -276,8 +277,9 @@ int tl_cheby_run(const double *ch_alphas, const double *ch_betas, int step_first
  * and adds u += d in FP64.  It stops when ||u0 - A u|| <= rel_tol times the
  * residual of the starting u (a true residual, unlike the recurrence
  * residual of tea_solve.f90:402-410), after max_outer refinements, or when
- * a refinement fails to halve the true residual (it has reached its
- * rounding floor; reason TL_LOOP_FLOOR).  Requires
+ * the true residual has reached its rounding floor (reason
+ * TL_LOOP_FLOOR): a refinement whose inner solve reduced its own residual
+ * at least 4-fold, or two in a row, failed to halve it.  Requires
  * tl_preconditioner_type none and a chunk with all four faces external. */
 #define TL_MIXED_CG 1
 #define TL_MIXED_PPCG 2

@@ -362,12 +362,18 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
   int pcur = 0, sdcur = 0;
   LoopBufs<float> b = f32_bufs(pcur, sdcur);
   double res = r0;
-  // A refinement that does not halve the true residual has met its rounding
-  // floor (a target below it, e.g. relative to a small initial residual,
-  // cannot be met): the solve ends there.  Another try from the same
-  // residual measured as slow and no better (8192^2 adjoint steps).
+  // The true residual's rounding floor: a target below it (e.g. relative to
+  // a small initial residual) cannot be met.  A refinement whose inner solve
+  // reduced its own residual at least 4-fold while the true residual did not
+  // halve has reached it: the solve ends there (another refinement on a
+  // residual of rounding noise costs hundreds of inner iterations for
+  // nothing).  A refinement whose inner solve gained less, such as the
+  // eigenvalue-estimate CG of PPCG's first refinement, which stops when the
+  // estimate settles, counts as a stall; two in a row end the solve too.
+  int stalls = 0;  // refinements in a row that did not halve the true residual
   bool floor = false;
-  while (res > target && st->outer < st->max_outer && !floor) {
+  while (res > target && st->outer < st->max_outer && stalls < 2 && !floor) {
+    double inner_gain = 0.0;  // the inner solve's start / end residual
     if ((rc = zero_d())) return rc;
     // ask an inner solve for no more than the remaining reduction (with a
     // margin), and never less than inner_tol
@@ -401,6 +407,7 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
                                         ls.n_first == 1 ? b.r : nullptr)))
           return rc;
         st->inner_iters += ls.n_end - ls.n_first + 1;
+        inner_gain = sqrt(rr32) / std::max(ls.error, 1e-300);
         if (!estimate) break;
         eig_ok = ls.n_end >= 2 && lanczos_bounds(al, be, ls.n_end, &emin, &emax);
         st->eig_iters = ls.n_end;
@@ -440,14 +447,21 @@ extern "C" int tl_mixed_solve(tl_mixed_state *st) {
         return rc;
       st->inner_iters += ls.n_end;
       st->sweeps += sweeps;
+      inner_gain = sqrt(zr) / std::max(ls.error, 1e-300);
     }
     if ((rc = refine(true, &rr64, &rr32))) return rc;
     const double prev = res;
     res = sqrt(rr64);
     st->outer += 1;
-    floor = res > 0.5 * prev;
+    if (res > 0.5 * prev) {
+      stalls += 1;
+      floor = inner_gain >= 4.0;
+    } else {
+      stalls = 0;
+    }
     st->final_residual = res;
   }
-  st->reason = res <= target ? TL_LOOP_CONVERGED : (floor ? TL_LOOP_FLOOR : TL_LOOP_MAXITERS);
+  st->reason = res <= target ? TL_LOOP_CONVERGED
+                             : ((floor || stalls >= 2) ? TL_LOOP_FLOOR : TL_LOOP_MAXITERS);
   return TL_OK;
 }

@@ -228,3 +228,25 @@ def test_reflective_boundary_matches_reference(problems):
     err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
     print(f"  max rel gradient error vs reference {err:.2e}", flush=True)
     assert err <= 1e-9
+
+
+def test_mixed_ppcg_adjoint_at_scale():
+    """The bench's 2048^2 adjoint case (20 steps, checkpoint every 5) with
+    PPCG solves: the mixed gradient within 1e-8 of the FP64 one.  Later steps
+    start from residuals near the true residual's rounding floor, where a
+    mixed solve must tell a stall of the eigenvalue-estimate CG (which ends
+    when its estimate settles) from the floor itself; ending at the first
+    stall put this gradient 1.5e-3 off."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    import bench
+    print("adjoint 2048^2 PPCG, fp64 and mixed", flush=True)
+    out = bench.adjoint_bench(2048, 20, 5, 0, 1e-6, "both", solver="ppcg")
+    rel = out["mixed_vs_fp64_gradient_rel_l2"]
+    dj = abs(out["mixed"]["J"] - out["fp64"]["J"]) / abs(out["fp64"]["J"])
+    print(f"  mixed vs FP64 gradient rel L2 {rel:.2e}, J rel {dj:.2e}", flush=True)
+    assert rel <= 1e-8
+    assert dj <= 1e-9
